@@ -1,0 +1,200 @@
+/*
+ * tvbx: C ABI of the MI355X (gfx950) delayed-coupling network integrator.
+ *
+ * This is the drop-in boundary behind tvb_hpc's
+ * transforms.network_time_step(model, coupling, scheme, target)
+ * (reference: tvb_hpc/transforms.py:23-49) whose returned kernel is called as
+ *   knl(nstep, nnode, ntime, state, input, param, drift, diffs, obsrv, nnz,
+ *       delays, row, col, weights, <cfun params>, i_step_0)
+ * (tvb_hpc/examples/hackathon.py:49-53).  In the reference the kernel is a
+ * loopy-generated function reached through ctypes
+ * (tvb_hpc/compiler.py:140-184, CompiledKernel: argtypes from the generated
+ * header, arrays as T* restrict, scalars by value, void return).  Here the
+ * Python host (tvb_hpc/hip.py) binds these functions with ctypes instead.
+ *
+ * Conventions
+ *   - every function returns an int status: 0 (TVBX_OK) or a negative code;
+ *     nothing throws across the ABI; tvbx_last_error() returns the calling
+ *     thread's last message.  (The reference has no status at all:
+ *     compiler.py:160-164.)
+ *   - plain pointers and sizes only; host arrays are borrowed for the
+ *     duration of the call; device buffers are owned by the tvbx objects.
+ *   - arrays use the instance-innermost layout documented in
+ *     tvb-hpc_amd/csrc/tvbx_args.h; host-side arrays are unpadded (last axis
+ *     n_inst), device-side arrays are padded to a multiple of 64.
+ *   - calls on one context are asynchronous on the context's stream until a
+ *     call that returns host data (tvbx_sim_get_*), tvbx_ctx_sync, or
+ *     tvbx_sim_fc; one context per host thread.
+ */
+#ifndef TVBX_H
+#define TVBX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TVBX_ABI_VERSION 1
+
+#define TVBX_OK 0
+#define TVBX_E_ARG (-1)      /* invalid argument (validated before any launch) */
+#define TVBX_E_HIP (-2)      /* HIP runtime error */
+#define TVBX_E_COMPILE (-3)  /* hipRTC compilation failed */
+#define TVBX_E_STATE (-4)    /* call not valid in the object's current state */
+#define TVBX_E_NOMEM (-5)    /* device allocation failed */
+
+typedef struct tvbx_ctx tvbx_ctx;
+typedef struct tvbx_module tvbx_module;
+typedef struct tvbx_sim tvbx_sim;
+
+/* ---- errors / versioning ------------------------------------------------
+ * Reference: errors surface as Python exceptions raised by
+ * subprocess.check_call in Compiler._call (tvb_hpc/compiler.py:104-110) and
+ * ValueError in CompiledKernel (compiler.py:164, 214). */
+int tvbx_abi_version(void);
+int tvbx_last_error(char *buf, size_t len);
+
+/* ---- context ---------------------------------------------------------------
+ * Reference: the loopy target object (utils.default_target, utils.py:32) and
+ * PyCUDA's implicit context (dsl/dsl_cuda/example/cuda_run.py:58-63). */
+int tvbx_device_count(int *n);
+int tvbx_ctx_create(int device, tvbx_ctx **ctx);
+int tvbx_ctx_destroy(tvbx_ctx *ctx);
+int tvbx_ctx_sync(tvbx_ctx *ctx);
+/* name (gcnArchName), compute units and clock of the context's device */
+int tvbx_ctx_device_info(tvbx_ctx *ctx, char *arch, size_t arch_len,
+                         int *n_cu, int *clock_khz);
+
+/* ---- modules: emitted model kernels --------------------------------------
+ * Reference: Compiler.build (compiler.py:112-122) compiles generated C with
+ * gcc into a .so; NumbaTarget.get_kernel_executor (numba.py:21-32) execs
+ * generated Numba source; cuda_run.make_kernel (cuda_run.py:15-45) builds
+ * CUDA with PyCUDA.  Here: hipRTC on emitted HIP source, or a code object
+ * compiled ahead of time from the same source. */
+int tvbx_module_compile(tvbx_ctx *ctx, const char *source,
+                        const char *options, tvbx_module **out, char *log,
+                        size_t log_len);
+int tvbx_module_load(tvbx_ctx *ctx, const void *image, size_t size,
+                     tvbx_module **out);
+/* copies the module's code object; *size receives its size (buf may be
+ * null to query) */
+int tvbx_module_image(tvbx_module *mod, void *buf, size_t len, size_t *size);
+int tvbx_module_destroy(tvbx_module *mod);
+
+/* ---- simulation: one batch of sweep instances ---------------------------
+ * Reference: the arrays of the fused kernel call (hackathon.py:45-53:
+ * state/input/param/drift/diffs/obsrv + CSR nnz/delays/row/col/weights) and
+ * the CUDA sweep buffers (cuda_run.py:74-80: params (n_params, n_work_items),
+ * state (buf_len, S*N, n_work_items), tavg (N, n_work_items)). */
+typedef struct tvbx_sim_desc {
+  int n_node;          /* N */
+  int n_inst;          /* P */
+  int n_svar;          /* S: state variables of the model */
+  int n_ring;          /* R: observables kept per ring slot */
+  int n_coupled;       /* C: coupled observables (<= R) */
+  int horizon;         /* H: ring slots; must exceed max delay + 1 */
+  int nnz;             /* edges */
+  int n_group;         /* delay groups (e.g. conduction speeds) */
+  int group_size;      /* instances per delay group, a multiple of 64 */
+  int n_pinst, n_pnode, n_pni, n_pglob; /* parameter counts by kind */
+  float dt;            /* integration step */
+  int scheme;          /* 0 Euler, 1 Euler-Maruyama */
+  int tavg_period;     /* monitor period in steps; 0 disables the monitor */
+  int bold;            /* 1: monitor samples pass through Balloon-Windkessel */
+  int bold_obs;        /* observable index fed to the monitor */
+  int fc;              /* 1: accumulate FP64 FC of the monitor samples */
+  int sample_slots;    /* monitor sample buffer length */
+  int waves;           /* waves per workgroup */
+  int node_block;      /* nodes per workgroup (>= n_node: one group holds all) */
+  int steps_per_launch;/* > 1 only when node_block >= n_node */
+  int compat_outputs;  /* 1: keep last step's input/drift/diffs */
+  int order;           /* 0: step n reads y(n) written by step n-1 (the
+                        *    dsl_cuda kernels, kuramotoref.c:86-113);
+                        * 1: the loopy kernel's order (transforms.py:36-42):
+                        *    slot n is read for post_syn and zero delays
+                        *    before step n writes y(n) into it.  Must match
+                        *    the module's emitted order. */
+} tvbx_sim_desc;
+
+enum { TVBX_P_GLOB = 0, TVBX_P_INST = 1, TVBX_P_NODE = 2, TVBX_P_NI = 3 };
+
+int tvbx_sim_create(tvbx_ctx *ctx, tvbx_module *mod,
+                    const tvbx_sim_desc *desc, tvbx_sim **out);
+int tvbx_sim_destroy(tvbx_sim *sim);
+
+/* CSR connectome (network.py:66-78 Connectivity.from_dense: row = target,
+ * col = source) with integer delays per group, delays[n_group][nnz]
+ * (hackathon.py:44 computes them from lengths / speed / dt). Validates
+ * row[N] == nnz, col < N, 0 <= delay <= horizon - 2. */
+int tvbx_sim_set_connectome(tvbx_sim *sim, const int32_t *row,
+                            const int32_t *col, const float *weight,
+                            const int32_t *delays);
+/* parameters of one kind (TVBX_P_*): GLOB values[n]; INST values[n][P];
+ * NODE values[n][N]; NI values[n][N][P]  (model.py:67-84 prep_arrays;
+ * tmpl8_CUDA.py:58 params(i) per work item). */
+int tvbx_sim_set_params(tvbx_sim *sim, int kind, const float *values,
+                        size_t count);
+/* state x[S][N][P] (model.py:75-80; cuda_run.py:79) */
+int tvbx_sim_set_state(tvbx_sim *sim, const float *x);
+int tvbx_sim_get_state(tvbx_sim *sim, float *x);
+/* observable ring ring[H][R][N][P] (the reference's obsrv[ntime][nnode][nobs]
+ * with the instance axis appended) */
+int tvbx_sim_set_ring(tvbx_sim *sim, const float *ring);
+int tvbx_sim_get_ring(tvbx_sim *sim, float *ring);
+/* global step index n (the reference's i_step_0); ring slot = n mod H */
+int tvbx_sim_set_step(tvbx_sim *sim, long long step);
+int tvbx_sim_get_step(tvbx_sim *sim, long long *step);
+/* fill the ring from the state: slot (n mod H) = h(x); the other slots
+ * zero (mode 0, the reference's zeros obsrv buffer) or h(x) (mode 1).
+ * mode 2 writes only slot (n mod H). */
+int tvbx_sim_init_history(tvbx_sim *sim, int mode);
+/* noise source for Euler-Maruyama: 0 none, 1 Philox4x32-10 with the given
+ * 64-bit seed (counter = step, node | svar/4 << 24, instance, step >> 32) */
+int tvbx_sim_set_noise(tvbx_sim *sim, int mode, unsigned long long seed);
+/* advance n_steps.  noise: optional injected normals (host), either
+ * [n_steps][S][N][P] (tstride = S*N*P) or [S][N][P] reused each step
+ * (tstride = 0). */
+int tvbx_sim_run(tvbx_sim *sim, long long n_steps, const float *noise,
+                 long long noise_tstride);
+/* last step's input[C][N][P], drift[S][N][P], diffs[S][N][P] (any may be
+ * null); requires compat_outputs */
+int tvbx_sim_get_outputs(tvbx_sim *sim, float *input, float *drift,
+                         float *diffs);
+/* monitor samples not yet folded into FC: out[count][N][P]; *first is the
+ * global sample index of out[0] */
+int tvbx_sim_get_samples(tvbx_sim *sim, float *out, long long *first,
+                         int *count);
+/* fold pending samples into FC and write correlations of instances
+ * [p0, p0 + np) as corr[np][N][N] (metric.py:111-139 CovToCorr);
+ * *n_samples receives the number of samples accumulated */
+int tvbx_sim_fc(tvbx_sim *sim, int p0, int np, float *corr,
+                long long *n_samples);
+/* restart FC accumulation (keeps the simulation state) */
+int tvbx_sim_fc_reset(tvbx_sim *sim);
+/* per-instance flag: 1 if a state variable became non-finite */
+int tvbx_sim_get_flags(tvbx_sim *sim, uint32_t *flags);
+/* device time of the step kernels since the last reset, measured with
+ * events on the simulation's stream, and the number of step launches */
+int tvbx_sim_timing(tvbx_sim *sim, double *ms, long long *launches,
+                    int reset);
+
+/* ---- RNG ------------------------------------------------------------------
+ * Reference: RNG.fill (tvb_hpc/rng.py:63-96) fills a float32 array with
+ * Philox4x32 + Box-Muller normals.  Counters (i, 0, 0, 0) give out[4i..4i+3]. */
+int tvbx_rng_normals(tvbx_ctx *ctx, unsigned long long seed, float *out,
+                     size_t n);
+
+/* ---- metric kernels ---------------------------------------------------------
+ * Reference: OnlineCov / CovToCorr (tvb_hpc/metric.py:87-139), update_cov /
+ * cov_to_corr (dsl/dsl_cuda/CUDAmodels/refs/covar.c:11-109).  samples
+ * [n][N][P] (host) -> corr [P][N][N] (host), FP64 accumulation. */
+int tvbx_corr(tvbx_ctx *ctx, const float *samples, long long n, int N, int P,
+              float *corr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TVBX_H */

@@ -1,0 +1,304 @@
+"""Parity of the HIP integrator (called through libtvbx's C ABI) with the
+CPU oracle, on the device.
+
+Bar: float32 trajectories within 1e-5 of the float32 oracle, relative to each
+state variable's scale (``harness.state_err``; Kuramoto phases modulo 2 pi,
+relative to 2 pi), over 1000 steps with identical injected noise.  The two
+sides differ only in the last-bit rounding of the transcendental functions
+(libm vs. the device math library).
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from harness import (CASES, CFUNS, MODELS, oracle_for, scaled_err, small_csr,
+                     state_err)
+from tvb_hpc import scheme as SC
+from tvb_hpc.hip import Simulation, TvbxError
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _pair(model, cfun, dt, cparams, P=96, N=24, steps=1000, em=True,
+          launch=None, ring_pre=None, init='zeros', seed_noise=11,
+          order='cuda', zero_delays=0):
+    csr = small_csr(N=N, dt=dt, zero_delays=zero_delays)
+    m = MODELS[model]()
+    c = CFUNS[cfun](m)
+    scheme = SC.EulerMaryuyamaStep(dt) if em else SC.EulerStep(dt)
+    sigma = 0.01 if em else None
+    params = {'c:' + k: np.full(P, v, np.float32) if k == 'a' else v
+              for k, v in cparams.items()}
+    if em:
+        params['sigma'] = np.full(P, sigma, np.float32)
+    sim = Simulation(m, c, scheme, csr, P, params=params,
+                     kinds={'sigma': 'inst'} if em else None,
+                     ring_pre=ring_pre, order=order, **(launch or {}))
+    rng = np.random.default_rng(seed_noise)
+    x0 = m.initial_state(N, P, rng)
+    sim.set_state(x0)
+    sim.init_history(init)
+    ref = oracle_for(model, cfun, csr, P, sim.H, dt, 'em' if em else 'euler',
+                     cparams=cparams, sigma=sigma, order=order)
+    ref.set_state(x0)
+    ref.init_history(init)
+    S = len(m.state_sym)
+    noise = rng.standard_normal((steps, S, N, P)).astype(np.float32) \
+        if em else None
+    return sim, ref, noise
+
+
+LAUNCHES = [dict(waves=4, node_block=4, steps_per_launch=1),
+            dict(waves=8, node_block=10 ** 6, steps_per_launch=100)]
+
+
+@pytest.mark.parametrize('model,cfun,dt,cparams', CASES)
+@pytest.mark.parametrize('launch', LAUNCHES)
+def test_trajectory_parity_injected_noise(model, cfun, dt, cparams, launch):
+    sim, ref, noise = _pair(model, cfun, dt, cparams, launch=launch)
+    for k0 in range(0, noise.shape[0], 250):
+        sim.run(250, noise=noise[k0:k0 + 250])
+        ref.run(250, noise=noise[k0:k0 + 250])
+        xg, xr = sim.get_state(), ref.get_state()
+        assert np.isfinite(xg).all()
+        err = state_err(model, xg, xr)
+        assert err < TOL, (model, cfun, k0 + 250, err)
+    assert not sim.flags().any()
+
+
+@pytest.mark.parametrize('model,cfun,dt,cparams',
+                         [c for c in CASES if c[1] in ('Kuramoto', 'Diff')])
+@pytest.mark.parametrize('launch', LAUNCHES)
+def test_reference_order_parity(model, cfun, dt, cparams, launch):
+    """The loopy kernel's order (post_syn and zero delays read the slot
+    before it is rewritten), with some zero-delay edges."""
+    sim, ref, noise = _pair(model, cfun, dt, cparams, launch=launch,
+                            order='reference', zero_delays=12)
+    sim.run(600, noise=noise[:600])
+    ref.run(600, noise=noise[:600])
+    assert state_err(model, sim.get_state(), ref.get_state()) < TOL
+    # the ring as the reference's obsrv array would hold it
+    assert scaled_err(sim.get_ring(), ref.get_ring()) < TOL
+
+
+@pytest.mark.parametrize('model,cfun,dt,cparams', CASES[:4])
+def test_trajectory_parity_deterministic(model, cfun, dt, cparams):
+    """Without noise; 500 steps, because the delayed Kuramoto network is
+    chaotic enough deterministically that last-bit differences of sinf grow
+    past the bar by step 1000 (8e-5 of 2 pi)."""
+    sim, ref, _ = _pair(model, cfun, dt, cparams, em=False,
+                        launch=dict(waves=2, node_block=3), zero_delays=5)
+    sim.run(500)
+    ref.run(500)
+    assert state_err(model, sim.get_state(), ref.get_state()) < TOL
+
+
+def test_ring_contents_match_oracle():
+    sim, ref, noise = _pair('Generic2dOscillator', 'Sigmoidal', 0.1,
+                            {'a': 0.5}, ring_pre=False, steps=300)
+    sim.run(300, noise=noise)
+    ref.run(300, noise=noise)
+    rg, rr = sim.get_ring(), ref.get_ring()
+    assert rg.shape == rr.shape
+    assert scaled_err(rg, rr) < TOL
+
+
+def test_philox_mode_matches_oracle():
+    """Counter-based noise: the same (step, node, instance, svar) counters on
+    both sides; normals differ only by the rounding of log / sin / cos."""
+    sim, ref, _ = _pair('Generic2dOscillator', 'Sigmoidal', 0.1, {'a': 0.5})
+    sim.set_noise(seed=12345)
+    sim.run(1000)
+    ref.run(1000, seed=12345)
+    assert scaled_err(sim.get_state(), ref.get_state()) < TOL
+
+
+def test_rng_normals_match_oracle(ctx):
+    z = ctx.rng_normals(1 << 16, seed=99)
+    zr = O.normals_f32(1 << 16, seed=99)
+    np.testing.assert_allclose(z, zr, rtol=2e-6, atol=2e-6)
+
+
+def test_instance_independence_and_determinism():
+    """Instance p computes the same bits whatever else runs beside it, and
+    two runs from the same state agree bit for bit."""
+    model, cfun, dt, cp = 'JansenRit', 'Sigmoidal', 0.05, {'a': 0.3}
+    simA, _, noise = _pair(model, cfun, dt, cp, P=200, steps=200)
+    x0 = simA.get_state()
+    simA.run(200, noise=noise)
+    xa = simA.get_state()
+    simB, _, _ = _pair(model, cfun, dt, cp, P=64, steps=200)
+    simB.set_state(x0[..., :64])
+    simB.init_history('zeros')
+    simB.run(200, noise=noise[..., :64])
+    np.testing.assert_array_equal(simB.get_state(), xa[..., :64])
+    simA.set_state(x0)
+    simA.step = 0
+    simA.init_history('zeros')
+    simA.run(200, noise=noise)
+    np.testing.assert_array_equal(simA.get_state(), xa)
+
+
+def test_delay_groups_match_oracle():
+    """Several conduction speeds: one delay row per group of 64 instances."""
+    dt = 0.1
+    csr = small_csr(N=20, dt=dt, speeds=(1.0, 2.5, 7.0))
+    P, gs = 192, 64
+    m = MODELS['Generic2dOscillator']()
+    c = CFUNS['Sigmoidal'](m)
+    G = np.linspace(0, 1, P).astype(np.float32)
+    sim = Simulation(m, c, SC.EulerStep(dt), csr, P, params={'c:a': G},
+                     group_size=gs, waves=4, node_block=5)
+    x0 = m.initial_state(20, P, np.random.default_rng(3))
+    sim.set_state(x0)
+    sim.init_history('zeros')
+    ref = oracle_for('Generic2dOscillator', 'Sigmoidal', csr, P, sim.H, dt,
+                     'euler', cparams={'a': G}, group_size=gs)
+    ref.set_state(x0)
+    ref.init_history('zeros')
+    sim.run(800)
+    ref.run(800)
+    assert scaled_err(sim.get_state(), ref.get_state()) < TOL
+
+
+def _monitor_pair(fc, slots):
+    dt = 0.1
+    csr = small_csr(N=16, dt=dt)
+    P = 64
+    m = MODELS['Generic2dOscillator']()
+    c = CFUNS['Sigmoidal'](m)
+    sc = SC.EulerMaryuyamaStep(dt, sigma=0.005)
+    G = np.linspace(0, 1, P).astype(np.float32)
+    sim = Simulation(m, c, sc, csr, P, params={'c:a': G}, tavg_period=20,
+                     bold=True, fc=fc, sample_slots=slots, waves=4)
+    x0 = m.initial_state(16, P, np.random.default_rng(5))
+    sim.set_state(x0)
+    sim.init_history('zeros')
+    sim.set_noise(seed=7)
+    ref = oracle_for('Generic2dOscillator', 'Sigmoidal', csr, P, sim.H, dt,
+                     'em', cparams={'a': G}, sigma=0.005, tavg_period=20,
+                     bold=True)
+    ref.set_state(x0)
+    ref.init_history('zeros')
+    return sim, ref
+
+
+def test_monitor_bold_samples_match_oracle():
+    """tavg -> Balloon-Windkessel samples of the device match the oracle's,
+    with identical injected noise.  (With Philox noise the normals differ in
+    their last bit -- device sincospif vs. host cos -- and a few excitable
+    instances of this network amplify that into different spikes.)"""
+    sim, ref = _monitor_pair(fc=False, slots=100)
+    noise = np.random.default_rng(9).standard_normal(
+        (2000, 2, 16, 64)).astype(np.float32)
+    sim.run(2000, noise=noise)
+    out = ref.run(2000, noise=noise)
+    s_dev, first = sim.samples()
+    assert first == 0 and s_dev.shape == (100, 16, 64)
+    s_ref = out['samples']
+    err = np.array([scaled_err(s_dev[..., p], s_ref[..., p])
+                    for p in range(64)])
+    print('worst sample error per instance:', np.sort(err)[-4:],
+          'at', np.argsort(err)[-4:])
+    assert err.max() < 1e-4
+
+
+def test_device_fc_matches_fp64_fc_of_samples():
+    """FP64 co-moments folded in blocks of 7 samples (Chan et al. merges)
+    give the correlation of the same samples computed by the oracle's
+    one-pass FP64 Welford and by numpy.corrcoef."""
+    sim, ref = _monitor_pair(fc=True, slots=7)
+    sim2, _ = _monitor_pair(fc=False, slots=100)
+    sim.run(2000)
+    sim2.run(2000)
+    fc = sim.fc()
+    assert sim.n_fc_samples == 100
+    s_dev, _ = sim2.samples()
+    s64 = s_dev.astype(np.float64)
+    fc_ref = O.fc_f64(s64)
+    assert np.abs(fc - fc_ref).max() < 2e-6
+    for p in (0, 17, 63):
+        np.testing.assert_allclose(np.corrcoef(s64[:, :, p].T), fc_ref[p],
+                                   atol=1e-9)
+
+
+def test_corr_kernel_matches_numpy(ctx):
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((300, 12, 70)).astype(np.float32)
+    x += 0.5 * x[:, :1]
+    c = ctx.corr(x)
+    for p in (0, 33, 69):
+        np.testing.assert_allclose(c[p], np.corrcoef(x[:, :, p].T), atol=2e-6)
+
+
+def test_validation_errors():
+    csr = small_csr(N=10)
+    m = MODELS['Linear']()
+    c = CFUNS['Linear'](m)
+    with pytest.raises(ValueError):
+        Simulation(m, c, SC.EulerStep(0.1), csr, 64,
+                   horizon=csr.max_delay + 1)
+    sim = Simulation(m, c, SC.EulerStep(0.1), csr, 64)
+    from tvb_hpc import hip
+    bad_row = csr.row.copy()
+    bad_row[-1] += 1
+    with pytest.raises(TvbxError, match='row'):
+        hip.check(hip.lib().tvbx_sim_set_connectome(
+            sim.h, bad_row.ctypes.data, csr.col.ctypes.data,
+            csr.weight.ctypes.data, csr.delays.ctypes.data))
+    big = csr.delays.copy()
+    big[0, 0] = sim.H - 1
+    with pytest.raises(TvbxError, match='horizon'):
+        hip.check(hip.lib().tvbx_sim_set_connectome(
+            sim.h, csr.row.ctypes.data, csr.col.ctypes.data,
+            csr.weight.ctypes.data, big.ctypes.data))
+    bad_col = csr.col.copy()
+    bad_col[3] = 10
+    with pytest.raises(TvbxError, match='col'):
+        hip.check(hip.lib().tvbx_sim_set_connectome(
+            sim.h, csr.row.ctypes.data, bad_col.ctypes.data,
+            csr.weight.ctypes.data, csr.delays.ctypes.data))
+    nanp = np.full(64, np.nan, np.float32)
+    with pytest.raises(TvbxError, match='finite'):
+        sim2 = Simulation(m, c, SC.EulerStep(0.1), csr, 64,
+                          params={'m:lam': nanp}, kinds={'m:lam': 'inst'})
+        del sim2
+
+
+def test_nonfinite_flag_marks_only_bad_instance():
+    dt = 0.1
+    csr = small_csr(N=12, dt=dt)
+    P = 64
+    m = MODELS['Linear']()
+    c = CFUNS['Linear'](m)
+    lam = np.full(P, -1.0, np.float32)
+    lam[5] = 1e30  # blows up instance 5 only
+    sim = Simulation(m, c, SC.EulerStep(dt), csr, P,
+                     params={'m:lam': lam, 'c:a': 0.01},
+                     kinds={'m:lam': 'inst'})
+    sim.set_state(np.ones((1, 12, P), np.float32))
+    sim.init_history('constant')
+    sim.run(50)
+    f = sim.flags()
+    assert f[5] == 1 and f.sum() == 1
+    x = sim.get_state()
+    assert np.isfinite(np.delete(x, 5, axis=2)).all()
+
+
+def test_reference_rww_singularity_is_flagged():
+    """The reference RWW's h = (a x - b) / (1 - exp(-d (a x - b))) is 0/0 when
+    a*x rounds to b; the device flags exactly the instances that hit it,
+    the others stay finite, and ReducedWongWang has no such point."""
+    sim, ref, _ = _pair('RWW', 'Linear', 0.1, {'a': 0.05}, em=False)
+    sim.run(1000)
+    x = sim.get_state()
+    f = sim.flags()
+    bad = ~np.isfinite(x).all(axis=(0, 1))
+    np.testing.assert_array_equal(f.astype(bool), bad)
+    sim2, _, _ = _pair('ReducedWongWang', 'Linear', 0.1, {'a': 0.05},
+                       em=False)
+    sim2.run(1000)
+    assert np.isfinite(sim2.get_state()).all() and not sim2.flags().any()

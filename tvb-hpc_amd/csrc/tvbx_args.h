@@ -1,0 +1,86 @@
+// Argument block of the emitted step kernels.
+//
+// Shared by the host runtime (tvbx_runtime.hip) and every kernel the emitter
+// writes (tvb_hpc/emit.py prepends this header to the generated source), so
+// the two sides agree on one layout.  Plain C: pointers, 32/64-bit integers
+// and floats only.
+//
+// Device layouts (instance axis innermost, padded to `pitch`, a multiple of
+// 64 so one wavefront covers 64 consecutive instances):
+//   state      [S][N][pitch]            f32
+//   ring       [H][R][N][pitch]         f32   (R observables per slot)
+//   p_inst     [n_pinst][pitch]         f32   per-instance parameters
+//   p_node     [n_pnode][N]             f32   per-node parameters
+//   p_ni       [n_pni][N][pitch]        f32   per-node-per-instance parameters
+//   p_glob     [n_pglob]                f32   run-time scalars
+//   coldelay   [n_group][nnz]           u32   (delay << 16) | col
+//   noise      [n_steps][S][N][pitch]   f32   injected noise (optional)
+//   tavg       [N][pitch]               f32   monitor accumulator
+//   bold_state [4][N][pitch]            f32   (s, f, v, q)
+//   samples    [sample_slots][N][pitch] f32   monitor output ring
+#ifndef TVBX_ARGS_H
+#define TVBX_ARGS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tvbx_step_args {
+  /* sizes */
+  int n_node;
+  int n_inst;
+  int pitch;
+  int horizon;
+  int n_ring;
+  int nnz;
+  int group_size;     /* instances per delay group (multiple of 64) */
+  int n_steps;        /* steps performed by this launch */
+  long long step0;    /* global index of the first step of this launch */
+  int slot0;          /* step0 mod horizon */
+  int node_block;     /* nodes per workgroup; grid.y = ceil(N / node_block) */
+  /* integration */
+  float dt;
+  float sqrt_dt;      /* sqrtf(dt) */
+  unsigned seed_lo;
+  unsigned seed_hi;
+  int noise_mode;     /* 0 none, 1 Philox, 2 injected */
+  int tavg_period;    /* monitor period in steps (0: off) */
+  long long noise_tstride; /* floats between steps of injected noise (0: reuse) */
+  const float *noise;
+  /* connectome */
+  const int *row;
+  const unsigned *coldelay;
+  const float *weight;
+  /* state */
+  float *state;
+  float *ring;
+  /* parameters */
+  const float *p_inst;
+  const float *p_node;
+  const float *p_ni;
+  const float *p_glob;
+  /* compatibility outputs: last step's input / drift / diffs (may be null) */
+  float *out_input;
+  float *out_drift;
+  float *out_diffs;
+  /* monitor: time average of one observable, optionally through BOLD */
+  float *tavg;
+  float *bold_state;
+  float *samples;
+  int sample_slots;
+  int bold;           /* 1: Balloon-Windkessel on the averages */
+  float bold_dt;      /* seconds */
+  int bold_obs;       /* observable averaged by the monitor */
+  /* per-instance flag set when a state value becomes non-finite */
+  unsigned *bad;
+  /* reference (loopy) ordering only: [2][R][N][pitch], by step parity, the
+   * content the slot of step n had before y(n) replaced it -- what step n
+   * reads for post_syn and for zero delays */
+  float *stale;
+} tvbx_step_args;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
