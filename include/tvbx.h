@@ -116,6 +116,11 @@ typedef struct tvbx_sim_desc {
                         *    slot n is read for post_syn and zero delays
                         *    before step n writes y(n) into it.  Must match
                         *    the module's emitted order. */
+  int ipl;             /* sweep instances per lane (1, 2, 4); must match the
+                        * module; group_size must be a multiple of 64 * ipl */
+  int ring_type;       /* ring element: 0 float32, 1 bfloat16, 2 float16;
+                        * must match the module */
+  float ring_scale;    /* reduced-precision rings store ring_scale * pre(y) */
 } tvbx_sim_desc;
 
 enum { TVBX_P_GLOB = 0, TVBX_P_INST = 1, TVBX_P_NODE = 2, TVBX_P_NI = 3 };
@@ -153,6 +158,15 @@ int tvbx_sim_init_history(tvbx_sim *sim, int mode);
 /* noise source for Euler-Maruyama: 0 none, 1 Philox4x32-10 with the given
  * 64-bit seed (counter = step, node | svar/4 << 24, instance, step >> 32) */
 int tvbx_sim_set_noise(tvbx_sim *sim, int mode, unsigned long long seed);
+/* execution mode of tvbx_sim_run: use_graph != 0 replays hipGraph-captured
+ * windows of graph_window steps (the step counter lives on the device, so
+ * one capture serves every window; runs with injected noise and remainders
+ * are launched directly); profile != 0 brackets every step launch (or graph
+ * window) with events for tvbx_sim_timing.  Default: graphs of 100 steps,
+ * no profiling.  (The reference's driver launches one kernel per window of
+ * n_inner_steps from the host, cuda_run.py:124-142.) */
+int tvbx_sim_set_exec(tvbx_sim *sim, int use_graph, int graph_window,
+                      int profile);
 /* advance n_steps.  noise: optional injected normals (host), either
  * [n_steps][S][N][P] (tstride = S*N*P) or [S][N][P] reused each step
  * (tstride = 0). */
@@ -179,6 +193,29 @@ int tvbx_sim_get_flags(tvbx_sim *sim, uint32_t *flags);
  * events on the simulation's stream, and the number of step launches */
 int tvbx_sim_timing(tvbx_sim *sim, double *ms, long long *launches,
                     int reset);
+
+/* ---- adjoint (gradients for inversion) -----------------------------------
+ * The reference has no gradient kernel; its inversion hooks are the symbolic
+ * EM log-density and model partials (tvb_hpc/scheme.py:42-48,
+ * tvb_hpc/model.py:61-65).  These two calls give reverse-mode gradients of a
+ * loss of the monitor samples through the delayed Euler-Maruyama recursion.
+ * Requirements: module emitted with the adjoint, order 0, float32 history,
+ * no BOLD stage, delays >= 1, additive (or no) noise from Philox.
+ *
+ * tvbx_sim_checkpoint_run: like tvbx_sim_run for n_steps (a multiple of
+ * window), keeping the state and history at every window boundary. */
+int tvbx_sim_checkpoint_run(tvbx_sim *sim, long long n_steps, int window);
+/* tvbx_sim_adjoint: dL/dsample for samples [sample0, sample0 + n_samples)
+ * as [n_samples][N][P] (host, or device memory when on_device) ->
+ * g_coupling [n_gc][P] (the coupling parameters of kind 'inst' in emitter
+ * order, summed over nodes) and g_model [n_gm][N][P] (model parameters of
+ * kind 'inst' / 'ni', per node), host or device memory (out_on_device).
+ * Replays each window forward from its checkpoint, then runs the reverse
+ * sweep; the simulation's end-of-run state is restored afterwards. */
+int tvbx_sim_adjoint(tvbx_sim *sim, const float *dsamples, int on_device,
+                     long long sample0, long long n_samples, int n_gc,
+                     float *g_coupling, int n_gm, float *g_model,
+                     int out_on_device);
 
 /* ---- RNG ------------------------------------------------------------------
  * Reference: RNG.fill (tvb_hpc/rng.py:63-96) fills a float32 array with

@@ -98,7 +98,21 @@ class OracleSpec(ctypes.Structure):
         ('samples_written', ctypes.c_longlong),
         ('n_threads', ctypes.c_int),
         ('order', ctypes.c_int), ('stale', ctypes.c_void_p),
+        ('ring_q', ctypes.c_int), ('ring_pre', ctypes.c_int),
+        ('ring_scale', ctypes.c_double),
     ]
+
+
+def quantize(x, ring_dtype):
+    "round-to-nearest-even to bfloat16 / float16, returned as float32"
+    x = np.asarray(x, np.float32)
+    if ring_dtype == 'f16':
+        return x.astype(np.float16).astype(np.float32)
+    if ring_dtype == 'bf16':
+        u = x.view(np.uint32).astype(np.uint64)
+        u = (u + 0x7fff + ((u >> 16) & 1)) & 0xffff0000
+        return u.astype(np.uint32).view(np.float32)
+    return x
 
 
 _lib = None
@@ -219,9 +233,15 @@ class OracleSim:
     def __init__(self, model, cfun, N, row, col, weight, delays, P, H,
                  dt, scheme='euler', params=None, cparams=None, sigma=None,
                  group_size=None, R=1, dtype=np.float64, tavg_period=0,
-                 bold=False, bold_obs=0, n_threads=0, order='cuda'):
+                 bold=False, bold_obs=0, n_threads=0, order='cuda',
+                 ring_dtype='f32', ring_pre=False, ring_scale=1.0):
         self.model, self.cfun = model, cfun
         self.order = {'cuda': 0, 'reference': 1}[order]
+        # emulation of a reduced-precision history (float32 oracle only)
+        self.ring_dtype = ring_dtype
+        self.ring_q = {'f32': 0, 'bf16': 1, 'f16': 2}[ring_dtype]
+        self.ring_pre = bool(ring_pre)
+        self.ring_scale = float(ring_scale)
         self.N, self.P, self.H, self.R = int(N), int(P), int(H), int(R)
         self.S = MODEL_NSVAR[model]
         self.dtype = dtype
@@ -294,6 +314,8 @@ class OracleSim:
         order 'reference' the slot's previous content is kept as what the
         step reads for post_syn and zero delays."""
         y = self.observe()  # (R, N, P)
+        if self.ring_q and not self.ring_pre:
+            y = quantize(y, self.ring_dtype).astype(self.dtype)
         yy = np.transpose(y, (2, 0, 1))  # (P, R, N)
         t = self.step % self.H
         if mode == 'zeros':
@@ -376,6 +398,9 @@ class OracleSim:
         sp.n_threads = self.n_threads
         sp.order = self.order
         sp.stale = _ptr(self.stale)
+        sp.ring_q = self.ring_q
+        sp.ring_pre = int(self.ring_pre)
+        sp.ring_scale = self.ring_scale
         f = lib().oracle_run_f64 if self.dtype == np.float64 else \
             lib().oracle_run_f32
         rc = f(ctypes.byref(sp))

@@ -78,6 +78,11 @@ typedef struct {
                                   //    step n rewrites it
   void *stale;                    // T[P][2][R][N] (order 1): old content of
                                   // the slot being rewritten, by step parity
+  int ring_q;                     // emulate a reduced-precision history:
+                                  // 0 none, 1 bfloat16, 2 float16
+  int ring_pre;                   // with ring_q: the history holds
+                                  // Q(ring_scale * pre(y)) (else Q(y))
+  double ring_scale;
 } OracleSpec;
 
 }  // extern "C"
@@ -135,6 +140,41 @@ inline T philox_normal(unsigned seed_lo, unsigned seed_hi, long long n, int i,
   if (q < 2) boxmuller<T>(o[0], o[1], z0, z1);
   else boxmuller<T>(o[2], o[3], z0, z1);
   return (q & 1) ? z1 : z0;
+}
+
+// ------------------------------------------------- reduced-precision values
+// round-to-nearest-even float32 -> bfloat16 / float16 -> float32, as the
+// device stores history elements
+
+inline float q_bf16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+
+inline float q_f16(float f) {
+  // exact via float16's grid: round the float32 to 11 significant bits
+  // (normal range) or to multiples of 2^-24 (subnormal range)
+  const float a = std::fabs(f);
+  if (!(a < 65520.0f)) return std::copysign(INFINITY, f);
+  if (a < 6.103515625e-05f) {  // 2^-14: subnormal spacing 2^-24
+    const float s = 5.9604644775390625e-08f;
+    return std::nearbyint(f / s) * s;  // default rounding mode: to even
+  }
+  int e;
+  std::frexp(a, &e);                       // a = m 2^e, m in [0.5, 1)
+  const float ulp = std::ldexp(1.0f, e - 11);
+  return std::nearbyint(f / ulp) * ulp;
+}
+
+template <typename T> inline T quant(int q, T v) {
+  if (q == 1) return T(q_bf16(float(v)));
+  if (q == 2) return T(q_f16(float(v)));
+  return v;
 }
 
 // ---------------------------------------------------------------- models
@@ -362,6 +402,9 @@ void run_model(OracleSpec *sp) {
   const BW<T> bw;
   const T bold_dt = T(sp->bold_dt);
   long long written = 0;
+  const bool qpre = sp->ring_q && sp->ring_pre;
+  const T rscale = T(float(sp->ring_scale));
+  const T runscale = T(1) / rscale;
 
 #pragma omp parallel for schedule(dynamic, 1) num_threads(sp->n_threads > 0 ? sp->n_threads : omp_get_max_threads()) reduction(max : written)
   for (int p = 0; p < P; ++p) {
@@ -396,8 +439,11 @@ void run_model(OracleSpec *sp) {
             T pre = (st_cur && dl[e] == 0)
                         ? st_cur[(size_t)kk * N + sp->col[e]]
                         : rg[((size_t)slot * R + kk) * N + sp->col[e]];
-            acc = std::fma(weight[e], cf_pre<T>(sp->cfun, pre, post, cp.data()), acc);
+            T term = cf_pre<T>(sp->cfun, pre, post, cp.data());
+            if (qpre) term = quant<T>(sp->ring_q, term * rscale);
+            acc = std::fma(weight[e], term, acc);
           }
+          if (qpre) acc = acc * runscale;
           c[kk] = cf_post<T>(sp->cfun, acc, N, cp.data());
         }
         M<T>::dfun(x, c, pv.data(), f, g);
@@ -422,7 +468,10 @@ void run_model(OracleSpec *sp) {
         for (int r = 0; r < R && r < NOBS; ++r) {
           T *dst = rg + ((size_t)t1 * R + r) * N + i;
           if (st_nxt) st_nxt[(size_t)r * N + i] = *dst;
-          *dst = y[r];
+          // a reduced-precision history stores Q(y) unless it holds pre(y),
+          // which is applied (and quantised) where it is read above
+          *dst = (sp->ring_q && !(sp->ring_pre && r < C)) ? quant<T>(sp->ring_q, y[r])
+                                                          : y[r];
         }
         if (traj)
           for (int s = 0; s < S; ++s)

@@ -1,0 +1,141 @@
+"""Gradients of an FC-fit loss through the delayed Euler-Maruyama recursion:
+the hand-written adjoint (HIP) against central finite differences of the
+float64 CPU oracle (h = 1e-3) -- the float64 reference gradient -- and
+against float32 device finite differences.
+
+Model: Montbrio-Pazo-Roxin with linear delayed coupling, additive Philox
+noise, tavg monitor of r; loss per instance 1 - corr(FC_sim, FC_target)
+with the target FC from a run at G = 0.5."""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from harness import small_csr
+from tvb_hpc import autograd as AG
+from tvb_hpc import coupling, model, scheme
+from tvb_hpc.hip import Simulation
+
+pytestmark = pytest.mark.gpu
+
+N, P, T, WIN, K, DT, SEED = 12, 64, 400, 100, 10, 0.1, 2024
+
+
+def _setup():
+    csr = small_csr(N=N, density=0.5, seed=11, speeds=(1.0, ), dt=DT,
+                    lmax=40.0)
+    assert csr.delays.min() >= 1
+    m = model.MontbrioPazoRoxin()
+    c = coupling.Linear(m)
+    sc = scheme.EulerMaryuyamaStep(DT, sigma=0.01)
+    rng = np.random.default_rng(5)
+    G = rng.uniform(0.2, 0.8, P).astype(np.float32)
+    eta = (-4.6 + 0.2 * rng.standard_normal((N, P))).astype(np.float32)
+    x0 = np.stack([rng.uniform(0.1, 1.0, (N, P)),
+                   rng.uniform(-1.5, -0.5, (N, P))]).astype(np.float32)
+    return csr, m, c, sc, G, eta, x0
+
+
+def _device_sim(csr, m, c, sc, G, eta):
+    return Simulation(m, c, sc, csr, P, params={'c:a': G, 'm:eta': eta},
+                      kinds={'c:a': 'inst', 'm:eta': 'ni'}, tavg_period=K,
+                      sample_slots=T // K, adjoint=True, waves=4,
+                      node_block=4)
+
+
+def _oracle_samples(csr, G, eta, x0):
+    sim = O.OracleSim('MontbrioPazoRoxin', 'Linear', N, csr.row, csr.col,
+                      csr.weight, csr.delays, P, csr.max_delay + 2, DT,
+                      scheme='em', params={'eta': eta}, cparams={'a': G},
+                      sigma=0.01, dtype=np.float64, tavg_period=K)
+    sim.set_state(x0)
+    sim.init_history('zeros')
+    return sim.run(T, seed=SEED)['samples']
+
+
+def _loss64(samples, fct):
+    s = torch.from_numpy(np.asarray(samples, np.float64))
+    return AG.fc_loss(s, fct).numpy()
+
+
+def test_adjoint_matches_fp64_finite_differences():
+    csr, m, c, sc, G, eta, x0 = _setup()
+    # target FC from a run at G = 0.5
+    s_t = _oracle_samples(csr, np.full(P, 0.5), eta, x0)
+    fct = AG.fc(torch.from_numpy(s_t))[0].numpy()
+
+    sim = _device_sim(csr, m, c, sc, G, eta)
+    Gt = torch.tensor(G, requires_grad=True)
+    et = torch.tensor(eta, requires_grad=True)
+    smp = AG.integrate(sim, {'c:a': Gt, 'm:eta': et}, x0, T, WIN, seed=SEED)
+    assert smp.shape == (T // K, N, P)
+    L = AG.fc_loss(smp, fct).sum()
+    L.backward()
+    gG = Gt.grad.numpy()
+    geta = et.grad.numpy()
+    assert np.isfinite(gG).all() and np.isfinite(geta).all()
+
+    # float64 reference gradient: central differences of the oracle
+    h = 1e-3
+    Lp = _loss64(_oracle_samples(csr, G.astype(np.float64) + h, eta, x0), fct)
+    Lm = _loss64(_oracle_samples(csr, G.astype(np.float64) - h, eta, x0), fct)
+    fdG = (Lp - Lm) / (2 * h)
+    errG = np.abs(gG - fdG).max() / np.abs(fdG).max()
+    print('dL/dG: adjoint vs fp64 central differences, rel err %.2e '
+          '(max |grad| %.3e)' % (errG, np.abs(fdG).max()))
+    assert errG < 1e-4
+    for i in (0, 5, 11):
+        e1 = eta.astype(np.float64).copy()
+        e2 = e1.copy()
+        e1[i] += h
+        e2[i] -= h
+        fd = (_loss64(_oracle_samples(csr, G, e1, x0), fct) -
+              _loss64(_oracle_samples(csr, G, e2, x0), fct)) / (2 * h)
+        err = np.abs(geta[i] - fd).max() / np.abs(fd).max()
+        print('dL/deta[%d]: rel err %.2e' % (i, err))
+        assert err < 1e-4
+
+
+def test_adjoint_matches_device_finite_differences():
+    """float32 central differences on the device (h = 1e-3) agree with the
+    adjoint to the precision float32 differences allow."""
+    csr, m, c, sc, G, eta, x0 = _setup()
+    s_t = _oracle_samples(csr, np.full(P, 0.5), eta, x0)
+    fct = AG.fc(torch.from_numpy(s_t))[0].numpy()
+    sim = _device_sim(csr, m, c, sc, G, eta)
+    Gt = torch.tensor(G, requires_grad=True)
+    et = torch.tensor(eta, requires_grad=True)
+    smp = AG.integrate(sim, {'c:a': Gt, 'm:eta': et}, x0, T, WIN, seed=SEED)
+    AG.fc_loss(smp, fct).sum().backward()
+    h = 1e-3
+
+    def dev_loss(Gv):
+        s = AG.integrate(sim, {'c:a': torch.tensor(Gv, dtype=torch.float64),
+                               'm:eta': torch.tensor(eta)}, x0, T, WIN,
+                         seed=SEED)
+        return AG.fc_loss(s, fct).detach().numpy()
+
+    fd = (dev_loss(G.astype(np.float64) + h) -
+          dev_loss(G.astype(np.float64) - h)) / (2 * h)
+    err = np.abs(Gt.grad.numpy() - fd).max() / np.abs(fd).max()
+    print('dL/dG: adjoint vs fp32 device central differences %.2e' % err)
+    assert err < 5e-3
+
+
+def test_adjoint_repeatable_and_state_restored():
+    csr, m, c, sc, G, eta, x0 = _setup()
+    sim = _device_sim(csr, m, c, sc, G, eta)
+    sim.set_state(x0)
+    sim.init_history('zeros')
+    sim.set_noise(SEED)
+    sim.checkpoint_run(T, WIN)
+    x_end = sim.get_state()
+    smp, first = sim.samples()
+    dS = np.random.default_rng(0).standard_normal(smp.shape).astype(np.float32)
+    g1 = sim.adjoint(dS, first)
+    np.testing.assert_array_equal(sim.get_state(), x_end)
+    g2 = sim.adjoint(dS, first)
+    for k in g1:
+        np.testing.assert_array_equal(g1[k], g2[k])
+    assert sim.step == T

@@ -59,8 +59,10 @@ def _hackathon_like(batched):
             nnz=csr.nnz, delays=delays, row=csr.row.astype(np.uintc),
             col=csr.col.astype(np.uintc), weights=csr.weight, a=a,
             i_step_0=i * 10)
-        ref.step = i * 10
-        ref.init_history('current')
+        if i == 0:
+            # the first call fills obsrv[0] from the state; later calls
+            # continue the oracle's own history
+            ref.init_history('current')
         ref.run(10)
         xr = ref.get_state()
         xg = np.transpose(state.reshape(N, 1, P), (1, 0, 2))

@@ -1,0 +1,127 @@
+// Reverse-mode (adjoint) sweep of the delayed Euler-Maruyama recursion.
+//
+// Forward (order 'cuda', one coupled observable y = h_0(x)):
+//   c_i(n)   = post(sum_i(n)),  sum_i(n) = sum_{e in row i} w_e pre(y_{j_e}(n - d_e))
+//   x_i(n+1) = x_i(n) + dt f(x_i(n), c_i(n), theta) + sqrt(dt) g xi_i(n)
+//   sample_m = (1/K) sum_{n = mK}^{mK+K-1} h_mon(x(n))
+// With lambda(n) = dL/dx(n) and mu_i(n) = dt sum_s lambda_{i,s}(n+1) df_s/dc,
+//   lambda_j(n) = lambda_j(n+1) + dt J_f(x_j(n))^T lambda_j(n+1)
+//               + [dpost/dsum pre'(y_j(n)) sum_{e: col_e = j} w_e mu_{i_e}(n + d_e)] dh_0/dx
+//               + (1/K) dL/dsample_{n/K, j} dh_mon/dx
+//   dL/dtheta_c += mu_i(n) dpost/dtheta_c(sum_i(n))      (coupling parameters)
+//   dL/dtheta_m += dt sum_s lambda_{i,s}(n+1) df_s/dtheta_m  (model parameters)
+// The sum over the edges leaving source j is a transposed (CSC) gather of mu
+// from a ring of future steps (n + d, d >= 1), so no atomics are needed and
+// each (node, instance) pair is owned by one lane.  The noise is additive, so
+// it does not enter the reverse sweep; x(n) and sum(n) come from a forward
+// replay of the window (rec_x / rec_sum).  Zero delays are excluded by the
+// runtime (they would need mu of the same step).
+//
+// One reverse step per launch when the node axis is split over workgroups;
+// several (descending, barrier-separated) when one workgroup holds all nodes.
+
+extern "C" __global__ void __launch_bounds__(1024)
+tvbx_adjoint(const tvbx_adj_args A) {
+  const tvbx_step_args &a = A.f;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwave = blockDim.x >> 6;
+  const int pb = (blockIdx.x * 64 + lane) * TVBX_IPL;
+  const int N = a.n_node;
+  const int H = a.horizon;
+  const size_t NP = (size_t)N * a.pitch;
+  const int group = __builtin_amdgcn_readfirstlane(
+      (blockIdx.x * 64 * TVBX_IPL) / a.group_size);
+  const unsigned *__restrict__ crow = A.crow + (size_t)group * a.nnz;
+  const float *__restrict__ cw = A.cw;
+  const int *__restrict__ ccol = A.ccol;
+  const float *__restrict__ mu_r = A.mu;
+  const int i_begin = blockIdx.y * a.node_block;
+  const int i_end = min(N, i_begin + a.node_block);
+  const float rK = a.tavg_period > 0 ? 1.0f / (float)a.tavg_period : 0.0f;
+
+  tvbx_lane_params lp[TVBX_IPL];
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q) tvbx_load_lane(lp[q], a, pb + q);
+
+  for (int k = 0; k < a.n_steps; ++k) {
+    const long long n = a.step0 - k;
+    const int t = (int)(n % H);
+    const size_t w = (size_t)(n - a.rec_step0);
+    for (int j0 = i_begin + wave; j0 < i_end; j0 += nwave) {
+      const int j = __builtin_amdgcn_readfirstlane(j0);
+      const size_t jp = (size_t)j * a.pitch + pb;
+      // mu of the targets of j's outgoing edges at steps n + d
+      float accm[TVBX_IPL];
+#pragma unroll
+      for (int q = 0; q < TVBX_IPL; ++q) accm[q] = 0.0f;
+      for (int e = ccol[j]; e < ccol[j + 1]; ++e) {
+        const unsigned c = crow[e];
+        int slot = t + (int)(c >> 16);
+        slot -= (slot >= H) ? H : 0;
+        float v[TVBX_IPL];
+        tvbx_ldf(mu_r + ((size_t)slot * N + (c & 0xffffu)) * a.pitch + pb, v);
+        const float we = cw[e];
+#pragma unroll
+        for (int q = 0; q < TVBX_IPL; ++q) accm[q] = fmaf(we, v[q], accm[q]);
+      }
+      float xs[TVBX_S][TVBX_IPL], ls[TVBX_S][TVBX_IPL], sm[TVBX_IPL];
+#pragma unroll
+      for (int s = 0; s < TVBX_S; ++s) {
+        tvbx_ldf(A.rec_x + (w * TVBX_S + s) * NP + jp, xs[s]);
+        tvbx_ldf(A.lam + s * NP + jp, ls[s]);
+      }
+      tvbx_ldf(A.rec_sum + w * TVBX_C * NP + jp, sm);
+      float mu_o[TVBX_IPL], lo[TVBX_S][TVBX_IPL];
+#pragma unroll
+      for (int q = 0; q < TVBX_IPL; ++q) {
+        const int p = pb + q;
+        float x[TVBX_S], lam[TVBX_S], cc[TVBX_C], jt[TVBX_S], muc[TVBX_C],
+            gm[TVBX_NGM], gc[TVBX_NGC], y[TVBX_NOBS], dhc[TVBX_S],
+            dhm[TVBX_S];
+#pragma unroll
+        for (int s = 0; s < TVBX_S; ++s) {
+          x[s] = xs[s][q];
+          lam[s] = ls[s][q];
+        }
+        cc[0] = tvbx_post(0, sm[q], lp[q], a);
+        tvbx_dfun_vjp(x, cc, lam, jt, muc, gm, lp[q], a, j, p);
+        const float mu = a.dt * muc[0];
+        tvbx_obsrv(x, y, lp[q], a, j, p);
+        const float aj =
+            tvbx_dpost_dsum(lp[q], a) * tvbx_dpre(y[0], lp[q], a) * accm[q];
+        float fj = 0.0f;
+        if (A.forcing != nullptr && a.tavg_period > 0) {
+          const long long m = n / a.tavg_period - A.sample0;
+          if (m >= 0 && m < A.n_forcing)
+            fj = A.forcing[((size_t)m * N + j) * a.pitch + p] * rK;
+        }
+        tvbx_dobsrv(0, x, dhc, lp[q], a, j, p);
+        tvbx_dobsrv(a.bold_obs, x, dhm, lp[q], a, j, p);
+#pragma unroll
+        for (int s = 0; s < TVBX_S; ++s)
+          lo[s][q] = lam[s] + a.dt * jt[s] + aj * dhc[s] + fj * dhm[s];
+        mu_o[q] = mu;
+        tvbx_dpost_dparam(sm[q], gc, lp[q], a);
+#if TVBX_HAS_NGC
+#pragma unroll
+        for (int g = 0; g < TVBX_NGC; ++g) {
+          float *d = A.g_c + ((size_t)g * N + j) * a.pitch + p;
+          *d = *d + mu * gc[g];
+        }
+#endif
+#if TVBX_HAS_NGM
+#pragma unroll
+        for (int g = 0; g < TVBX_NGM; ++g) {
+          float *d = A.g_m + ((size_t)g * N + j) * a.pitch + p;
+          *d = *d + a.dt * gm[g];
+        }
+#endif
+      }
+#pragma unroll
+      for (int s = 0; s < TVBX_S; ++s) tvbx_stf(A.lam + s * NP + jp, lo[s]);
+      tvbx_stf(A.mu + ((size_t)t * N + j) * a.pitch + pb, mu_o);
+    }
+    if (k + 1 < a.n_steps) __syncthreads();
+  }
+}

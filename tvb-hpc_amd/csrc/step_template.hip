@@ -1,32 +1,138 @@
 // Generic part of the emitted network step kernels (gfx950, wave64).
 //
 // The emitter (tvb_hpc/emit.py) writes, in front of this text:
-//   TVBX_S, TVBX_C, TVBX_NOBS, TVBX_R       state vars, coupled observables,
-//                                           observables, observables per ring slot
-//   TVBX_EM                                 1: Euler-Maruyama, 0: Euler
-//   TVBX_RING_PRE                           1: the ring holds pre(y) instead of y
-//   TVBX_USES_POST                          1: pre() reads post_syn
+//   TVBX_S, TVBX_C, TVBX_NOBS, TVBX_R   state vars, coupled observables,
+//                                       observables, observables per ring slot
+//   TVBX_EM          1: Euler-Maruyama, 0: Euler
+//   TVBX_RING_PRE    1: the ring holds pre(y) (scaled) instead of y
+//   TVBX_USES_POST   1: pre() reads post_syn
+//   TVBX_ORDER_REF   1: the loopy kernel's read-before-write order
+//   TVBX_IPL         sweep instances per lane (1, 2 or 4)
+//   TVBX_RING_T      ring element: 0 float32, 1 bfloat16, 2 float16
 //   struct tvbx_lane_params + tvbx_load_lane()   per-instance parameters
 //   tvbx_pre(), tvbx_post(), tvbx_dfun(), tvbx_obsrv(), tvbx_wrap(),
 //   tvbx_ring_value(), tvbx_bold_step()
 //
-// Work decomposition: one lane per sweep instance, 64 consecutive instances
-// per wavefront (blockIdx.x selects the instance tile), waves of a workgroup
-// take distinct nodes of the node block blockIdx.y.  Every lane of a wave
-// walks the same CSR row, so row / column / delay / weight loads are
-// wave-uniform (scalar loads) and each delayed-observable gather is one
-// contiguous 256-byte read of ring[slot][k][j][p0 .. p0+63].
+// Work decomposition: a wavefront covers 64 * TVBX_IPL consecutive sweep
+// instances (TVBX_IPL adjacent ones per lane); blockIdx.x selects that
+// instance tile, the waves of a workgroup take distinct nodes of node block
+// blockIdx.y.  All lanes of a wave walk the same CSR row, so row / column /
+// delay / weight loads are wave-uniform (scalar loads), and each delayed-
+// observable gather is one contiguous read of 64 * TVBX_IPL ring elements
+// ring[slot][k][j][p0 ...] (256 B .. 1 KiB).
 //
 // Step semantics (reference tvb_hpc/transforms.py:23-49 with the read/write
 // order of dsl/dsl_cuda/CUDAmodels/kuramotoref.c:80-115): at step n with ring
 // slot t = n mod H, node i reads post_syn = y_i(n) from slot t and
 // pre_syn = y_j(n - d) from slot (t - d) mod H, then writes x(n + 1) and
 // y(n + 1) = h(x(n + 1)) into slot (t + 1) mod H.  H >= max delay + 2 keeps
-// the write slot distinct from every slot read in the same step.
+// the write slot distinct from every slot read in the same step.  With
+// TVBX_ORDER_REF, post_syn and zero delays read what slot t held before
+// step n rewrote it (kept in a parity-double-buffered stale block).
+//
+// Precision: state, coupling sums, dfun and the update are float32; the
+// ring may be bfloat16 or float16 (round to nearest even), in which case it
+// stores ring_scale * pre(y) and the sum is multiplied back by ring_unscale.
 
 #ifndef TVBX_UNROLL
 #define TVBX_UNROLL 8
 #endif
+
+#if TVBX_RING_T == 0
+typedef float tvbx_rt;
+__device__ __forceinline__ float tvbx_r2f(tvbx_rt v) { return v; }
+__device__ __forceinline__ tvbx_rt tvbx_f2r(float v) { return v; }
+#elif TVBX_RING_T == 1
+typedef unsigned short tvbx_rt;
+__device__ __forceinline__ float tvbx_r2f(tvbx_rt v) {
+  return __uint_as_float((unsigned)v << 16);
+}
+__device__ __forceinline__ tvbx_rt tvbx_f2r(float v) {
+  // round to nearest even (inputs are finite or flagged elsewhere)
+  unsigned u = __float_as_uint(v);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (tvbx_rt)(u >> 16);
+}
+#else
+typedef _Float16 tvbx_rt;
+__device__ __forceinline__ float tvbx_r2f(tvbx_rt v) { return (float)v; }
+__device__ __forceinline__ tvbx_rt tvbx_f2r(float v) { return (_Float16)v; }
+#endif
+
+// TVBX_IPL consecutive float32 values
+__device__ __forceinline__ void tvbx_ldf(const float *__restrict__ p,
+                                         float (&o)[TVBX_IPL]) {
+#if TVBX_IPL == 4
+  const float4 v = *reinterpret_cast<const float4 *>(p);
+  o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+#elif TVBX_IPL == 2
+  const float2 v = *reinterpret_cast<const float2 *>(p);
+  o[0] = v.x; o[1] = v.y;
+#else
+  o[0] = *p;
+#endif
+}
+
+__device__ __forceinline__ void tvbx_stf(float *__restrict__ p,
+                                         const float (&v)[TVBX_IPL]) {
+#if TVBX_IPL == 4
+  *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+#elif TVBX_IPL == 2
+  *reinterpret_cast<float2 *>(p) = make_float2(v[0], v[1]);
+#else
+  *p = v[0];
+#endif
+}
+
+// TVBX_IPL consecutive ring elements as float32
+__device__ __forceinline__ void tvbx_ldr(const tvbx_rt *__restrict__ p,
+                                         float (&o)[TVBX_IPL]) {
+#if TVBX_RING_T == 0
+  tvbx_ldf(p, o);
+#else
+#if TVBX_IPL == 4
+  const uint2 v = *reinterpret_cast<const uint2 *>(p);
+  const unsigned w[2] = {v.x, v.y};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const unsigned short h = (unsigned short)(w[q >> 1] >> (16 * (q & 1)));
+    o[q] = tvbx_r2f(*reinterpret_cast<const tvbx_rt *>(&h));
+  }
+#elif TVBX_IPL == 2
+  const unsigned v = *reinterpret_cast<const unsigned *>(p);
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const unsigned short h = (unsigned short)(v >> (16 * q));
+    o[q] = tvbx_r2f(*reinterpret_cast<const tvbx_rt *>(&h));
+  }
+#else
+  o[0] = tvbx_r2f(*p);
+#endif
+#endif
+}
+
+__device__ __forceinline__ void tvbx_str(tvbx_rt *__restrict__ p,
+                                         const float (&v)[TVBX_IPL]) {
+#if TVBX_RING_T == 0
+  tvbx_stf(p, v);
+#else
+  unsigned short h[TVBX_IPL];
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q) {
+    const tvbx_rt r = tvbx_f2r(v[q]);
+    h[q] = *reinterpret_cast<const unsigned short *>(&r);
+  }
+#if TVBX_IPL == 4
+  *reinterpret_cast<uint2 *>(p) =
+      make_uint2((unsigned)h[0] | ((unsigned)h[1] << 16),
+                 (unsigned)h[2] | ((unsigned)h[3] << 16));
+#elif TVBX_IPL == 2
+  *reinterpret_cast<unsigned *>(p) = (unsigned)h[0] | ((unsigned)h[1] << 16);
+#else
+  *reinterpret_cast<unsigned short *>(p) = h[0];
+#endif
+#endif
+}
 
 // normals for state variables [s0, s0 + 4) of (instance p, step n, node i)
 __device__ __forceinline__ void tvbx_normals4(const tvbx_step_args &a,
@@ -52,19 +158,23 @@ __device__ __forceinline__ size_t tvbx_ring_off(int slot, int k, int j,
   return (size_t)((slot * TVBX_R + k) * a.n_node + j) * (size_t)a.pitch;
 }
 
-// y_j(n - d) of coupled observable kk.  In the reference (loopy) ordering a
-// zero delay reads what slot n held before step n rewrites it (st_cur).
-// d is wave-uniform, so the branch is a scalar one.
-__device__ __forceinline__ float tvbx_delayed(const float *__restrict__ ring,
-                                              const float *__restrict__ st_cur,
-                                              int t, int d, int kk, int j,
-                                              int p, const tvbx_step_args &a) {
+// y_j(n - d) of coupled observable kk for this lane's instances.  In the
+// reference (loopy) ordering a zero delay reads what slot n held before step
+// n rewrote it (st_cur).  d is wave-uniform: the branch is a scalar one.
+__device__ __forceinline__ void tvbx_delayed(const tvbx_rt *__restrict__ ring,
+                                             const tvbx_rt *__restrict__ st_cur,
+                                             int t, int d, int kk, int j,
+                                             int pb, const tvbx_step_args &a,
+                                             float (&o)[TVBX_IPL]) {
   int slot = t - d;
   slot += (slot < 0) ? a.horizon : 0;
 #if TVBX_ORDER_REF
-  if (d == 0) return st_cur[(size_t)(kk * a.n_node + j) * a.pitch + p];
+  if (d == 0) {
+    tvbx_ldr(st_cur + (size_t)(kk * a.n_node + j) * a.pitch + pb, o);
+    return;
+  }
 #endif
-  return ring[tvbx_ring_off(slot, kk, j, a) + p];
+  tvbx_ldr(ring + tvbx_ring_off(slot, kk, j, a) + pb, o);
 }
 
 extern "C" __global__ void __launch_bounds__(1024)
@@ -72,165 +182,235 @@ tvbx_step(const tvbx_step_args a) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwave = blockDim.x >> 6;
-  const int p = blockIdx.x * 64 + lane;
+  const int pb = (blockIdx.x * 64 + lane) * TVBX_IPL;
   const int N = a.n_node;
   const int H = a.horizon;
   const size_t NP = (size_t)N * a.pitch;
-  // delay group of this wavefront's 64 instances (group_size % 64 == 0)
-  const int group = __builtin_amdgcn_readfirstlane((blockIdx.x * 64) / a.group_size);
+  // delay group of this wavefront's instances (group_size % (64 IPL) == 0)
+  const int group = __builtin_amdgcn_readfirstlane(
+      (blockIdx.x * 64 * TVBX_IPL) / a.group_size);
   const unsigned *__restrict__ cd = a.coldelay + (size_t)group * a.nnz;
   const float *__restrict__ wt = a.weight;
   const int *__restrict__ row = a.row;
-  const float *__restrict__ ring = a.ring;
+  const tvbx_rt *__restrict__ ring = (const tvbx_rt *)a.ring;
+  tvbx_rt *ring_w = (tvbx_rt *)a.ring;
   const int i_begin = blockIdx.y * a.node_block;
   const int i_end = min(N, i_begin + a.node_block);
 
-  tvbx_lane_params lp;
-  tvbx_load_lane(lp, a, p);
+  tvbx_lane_params lp[TVBX_IPL];
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q) tvbx_load_lane(lp[q], a, pb + q);
   unsigned bad = 0u;
+  long long n0 = a.step0;
   int t = a.slot0;
+  if (a.clock != nullptr) {  // graph replay: offsets from the device clock
+    n0 += a.clock[0];
+    t = (int)((a.clock[1] + a.slot0) % H);
+  }
 
   for (int k = 0; k < a.n_steps; ++k) {
-    const long long n = a.step0 + k;
+    const long long n = n0 + k;
     const int t1 = (t + 1 == H) ? 0 : t + 1;
     const bool sample = a.tavg_period > 0 && ((n + 1) % a.tavg_period) == 0;
 #if TVBX_ORDER_REF
     const size_t RNP = (size_t)TVBX_R * NP;
-    const float *st_cur = a.stale + (size_t)(n & 1) * RNP;
-    float *st_nxt = a.stale + (size_t)((n + 1) & 1) * RNP;
+    const tvbx_rt *st_cur = (const tvbx_rt *)a.stale + (size_t)(n & 1) * RNP;
+    tvbx_rt *st_nxt = (tvbx_rt *)a.stale + (size_t)((n + 1) & 1) * RNP;
 #else
-    const float *st_cur = nullptr;
+    const tvbx_rt *st_cur = nullptr;
 #endif
     for (int i0 = i_begin + wave; i0 < i_end; i0 += nwave) {
       const int i = __builtin_amdgcn_readfirstlane(i0);
-      const size_t ip = (size_t)i * a.pitch + p;
-      float x[TVBX_S], f[TVBX_S], g[TVBX_S], c[TVBX_C], y[TVBX_NOBS];
+      const size_t ip = (size_t)i * a.pitch + pb;
+      float x[TVBX_IPL][TVBX_S], c[TVBX_IPL][TVBX_C];
 #pragma unroll
-      for (int s = 0; s < TVBX_S; ++s) x[s] = a.state[s * NP + ip];
+      for (int s = 0; s < TVBX_S; ++s) {
+        float v[TVBX_IPL];
+        tvbx_ldf(a.state + s * NP + ip, v);
+#pragma unroll
+        for (int q = 0; q < TVBX_IPL; ++q) x[q][s] = v[q];
+        if (a.rec_x != nullptr)
+          tvbx_stf(a.rec_x + ((size_t)(n - a.rec_step0) * TVBX_S + s) * NP + ip,
+                   v);
+      }
       // monitor input: observable of the state at the start of the step
-      float y_mon = 0.0f;
+      float y_mon[TVBX_IPL];
       if (a.tavg_period > 0) {
-        tvbx_obsrv(x, y, lp, a, i, p);
-        y_mon = tvbx_pick(y, a.bold_obs);
+#pragma unroll
+        for (int q = 0; q < TVBX_IPL; ++q) {
+          float y0[TVBX_NOBS];
+          tvbx_obsrv(x[q], y0, lp[q], a, i, pb + q);
+          y_mon[q] = tvbx_pick(y0, a.bold_obs);
+        }
       }
 
       // ---- delayed coupling: network.py:115-136
       const int e0 = row[i], e1 = row[i + 1];
 #pragma unroll
       for (int kk = 0; kk < TVBX_C; ++kk) {
-        float post = 0.0f;
+        float post[TVBX_IPL], acc[TVBX_IPL];
+#pragma unroll
+        for (int q = 0; q < TVBX_IPL; ++q) {
+          post[q] = 0.0f;
+          acc[q] = 0.0f;
+        }
+        if (TVBX_USES_POST) {
 #if TVBX_ORDER_REF
-        if (TVBX_USES_POST) post = st_cur[kk * NP + ip];
+          tvbx_ldr(st_cur + kk * NP + ip, post);
 #else
-        if (TVBX_USES_POST) post = ring[tvbx_ring_off(t, kk, i, a) + p];
+          tvbx_ldr(ring + tvbx_ring_off(t, kk, i, a) + pb, post);
 #endif
-        float acc = 0.0f;
+        }
         int e = e0;
         for (; e + TVBX_UNROLL <= e1; e += TVBX_UNROLL) {
-          float v[TVBX_UNROLL];
+          float v[TVBX_UNROLL][TVBX_IPL];
 #pragma unroll
           for (int u = 0; u < TVBX_UNROLL; ++u) {
             const unsigned w = cd[e + u];
-            v[u] = tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,
-                                (int)(w & 0xffffu), p, a);
+            tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,
+                         (int)(w & 0xffffu), pb, a, v[u]);
           }
 #pragma unroll
-          for (int u = 0; u < TVBX_UNROLL; ++u)
-            acc = fmaf(wt[e + u], tvbx_pre(kk, v[u], post, lp, a), acc);
+          for (int u = 0; u < TVBX_UNROLL; ++u) {
+            const float wu = wt[e + u];
+#pragma unroll
+            for (int q = 0; q < TVBX_IPL; ++q)
+              acc[q] = fmaf(wu, tvbx_pre(kk, v[u][q], post[q], lp[q], a),
+                            acc[q]);
+          }
         }
         for (; e < e1; ++e) {
           const unsigned w = cd[e];
-          const float vv = tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,
-                                        (int)(w & 0xffffu), p, a);
-          acc = fmaf(wt[e], tvbx_pre(kk, vv, post, lp, a), acc);
+          float vv[TVBX_IPL];
+          tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,
+                       (int)(w & 0xffffu), pb, a, vv);
+          const float we = wt[e];
+#pragma unroll
+          for (int q = 0; q < TVBX_IPL; ++q)
+            acc[q] = fmaf(we, tvbx_pre(kk, vv[q], post[q], lp[q], a), acc[q]);
         }
-        c[kk] = tvbx_post(kk, acc, lp, a);
+#pragma unroll
+        for (int q = 0; q < TVBX_IPL; ++q) {
+          acc[q] = acc[q] * a.ring_unscale;
+          c[q][kk] = tvbx_post(kk, acc[q], lp[q], a);
+        }
+        if (a.rec_sum != nullptr)
+          tvbx_stf(a.rec_sum + ((size_t)(n - a.rec_step0) * TVBX_C + kk) * NP +
+                       ip,
+                   acc);
       }
 
-      // ---- dfun: model.py:105-140
-      tvbx_dfun(x, c, f, g, lp, a, i, p);
-
-      // ---- scheme: scheme.py:82-97
+      float xo[TVBX_S][TVBX_IPL], yo[TVBX_R][TVBX_IPL];
+#pragma unroll
+      for (int q = 0; q < TVBX_IPL; ++q) {
+        const int p = pb + q;
+        float f[TVBX_S], g[TVBX_S], y[TVBX_NOBS];
+        // ---- dfun: model.py:105-140
+        tvbx_dfun(x[q], c[q], f, g, lp[q], a, i, p);
+        // ---- scheme: scheme.py:82-97
 #if TVBX_EM
-      float xi[TVBX_S];
-      if (a.noise_mode == 1) {
+        float xi[TVBX_S];
+        if (a.noise_mode == 1) {
 #pragma unroll
-        for (int s0 = 0; s0 < TVBX_S; s0 += 4) {
-          float z[4];
-          tvbx_normals4(a, n, i, p, s0, z);
+          for (int s0 = 0; s0 < TVBX_S; s0 += 4) {
+            float z[4];
+            tvbx_normals4(a, n, i, p, s0, z);
 #pragma unroll
-          for (int q = 0; q < 4; ++q)
-            if (s0 + q < TVBX_S) xi[s0 + q] = z[q];
+            for (int r = 0; r < 4; ++r)
+              if (s0 + r < TVBX_S) xi[s0 + r] = z[r];
+          }
+        } else if (a.noise_mode == 2) {
+          const float *nz = a.noise + (size_t)k * a.noise_tstride;
+#pragma unroll
+          for (int s = 0; s < TVBX_S; ++s) xi[s] = nz[s * NP + ip + q];
+        } else {
+#pragma unroll
+          for (int s = 0; s < TVBX_S; ++s) xi[s] = 0.0f;
         }
-      } else if (a.noise_mode == 2) {
-        const float *nz = a.noise + (size_t)k * a.noise_tstride;
 #pragma unroll
-        for (int s = 0; s < TVBX_S; ++s) xi[s] = nz[s * NP + ip];
-      } else {
-#pragma unroll
-        for (int s = 0; s < TVBX_S; ++s) xi[s] = 0.0f;
-      }
-#pragma unroll
-      for (int s = 0; s < TVBX_S; ++s)
-        x[s] = x[s] + a.dt * f[s] + a.sqrt_dt * xi[s] * g[s];
+        for (int s = 0; s < TVBX_S; ++s)
+          x[q][s] = x[q][s] + a.dt * f[s] + a.sqrt_dt * xi[s] * g[s];
 #else
 #pragma unroll
-      for (int s = 0; s < TVBX_S; ++s) x[s] = x[s] + a.dt * f[s];
+        for (int s = 0; s < TVBX_S; ++s) x[q][s] = x[q][s] + a.dt * f[s];
 #endif
-      tvbx_wrap(x);
-#pragma unroll
-      for (int s = 0; s < TVBX_S; ++s) {
-        a.state[s * NP + ip] = x[s];
-        bad |= (unsigned)!isfinite(x[s]);
-      }
-
-      // ---- history write: y(n + 1) into slot t + 1 (in the reference
-      // ordering the slot's previous content moves to the stale buffer that
-      // step n + 1 reads for post_syn and zero delays)
-      tvbx_obsrv(x, y, lp, a, i, p);
-#pragma unroll
-      for (int r = 0; r < TVBX_R; ++r) {
-        float *dst = a.ring + tvbx_ring_off(t1, r, i, a) + p;
-#if TVBX_ORDER_REF
-        st_nxt[r * NP + ip] = *dst;
-#endif
-        *dst = tvbx_ring_value(r, y[r], lp, a);
-      }
-
-      // ---- compatibility outputs of the last step of the launch
-      if (a.out_input != nullptr && k == a.n_steps - 1) {
-#pragma unroll
-        for (int kk = 0; kk < TVBX_C; ++kk) a.out_input[kk * NP + ip] = c[kk];
+        tvbx_wrap(x[q]);
 #pragma unroll
         for (int s = 0; s < TVBX_S; ++s) {
-          a.out_drift[s * NP + ip] = f[s];
-          a.out_diffs[s * NP + ip] = g[s];
+          xo[s][q] = x[q][s];
+          bad |= (unsigned)(!isfinite(x[q][s]) && p < a.n_inst) << q;
         }
+        // ---- observables of x(n + 1) for the history
+        tvbx_obsrv(x[q], y, lp[q], a, i, p);
+#pragma unroll
+        for (int r = 0; r < TVBX_R; ++r) {
+          float v = tvbx_ring_value(r, y[r], lp[q], a);
+          if (TVBX_RING_PRE && r < TVBX_C) v = v * a.ring_scale;
+          yo[r][q] = v;
+        }
+        // ---- compatibility outputs of the last step of the launch
+        if (a.out_input != nullptr && k == a.n_steps - 1) {
+#pragma unroll
+          for (int kk = 0; kk < TVBX_C; ++kk)
+            a.out_input[kk * NP + ip + q] = c[q][kk];
+#pragma unroll
+          for (int s = 0; s < TVBX_S; ++s) {
+            a.out_drift[s * NP + ip + q] = f[s];
+            a.out_diffs[s * NP + ip + q] = g[s];
+          }
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < TVBX_S; ++s) tvbx_stf(a.state + s * NP + ip, xo[s]);
+
+      // ---- history write: y(n + 1) into slot t + 1 (in the reference
+      // ordering the slot's previous content moves to the stale block that
+      // step n + 1 reads for post_syn and zero delays)
+#pragma unroll
+      for (int r = 0; r < TVBX_R; ++r) {
+        tvbx_rt *dst = ring_w + tvbx_ring_off(t1, r, i, a) + pb;
+#if TVBX_ORDER_REF
+        {
+          float old[TVBX_IPL];
+          tvbx_ldr(dst, old);
+          tvbx_str(st_nxt + r * NP + ip, old);
+        }
+#endif
+        tvbx_str(dst, yo[r]);
       }
 
       // ---- monitor: time average over tavg_period steps, optional BOLD
       if (a.tavg_period > 0) {
-        float acc_t = a.tavg[ip] + y_mon;
+        float acc_t[TVBX_IPL];
+        tvbx_ldf(a.tavg + ip, acc_t);
+#pragma unroll
+        for (int q = 0; q < TVBX_IPL; ++q) acc_t[q] = acc_t[q] + y_mon[q];
         if (sample) {
-          float out = acc_t / (float)a.tavg_period;
-          if (a.bold) out = tvbx_bold_step(a, ip, out);
+          float out[TVBX_IPL];
+#pragma unroll
+          for (int q = 0; q < TVBX_IPL; ++q) {
+            out[q] = acc_t[q] / (float)a.tavg_period;
+            if (a.bold) out[q] = tvbx_bold_step(a, ip + q, out[q]);
+            acc_t[q] = 0.0f;
+          }
           const long long m = (n + 1) / a.tavg_period - 1;
-          a.samples[(size_t)(m % a.sample_slots) * NP + ip] = out;
-          acc_t = 0.0f;
+          tvbx_stf(a.samples + (size_t)(m % a.sample_slots) * NP + ip, out);
         }
-        a.tavg[ip] = acc_t;
+        tvbx_stf(a.tavg + ip, acc_t);
       }
     }
     if (k + 1 < a.n_steps) __syncthreads();
     t = t1;
   }
-  if (bad && p < a.n_inst) a.bad[p] = 1u;
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q)
+    if ((bad >> q) & 1u) a.bad[pb + q] = 1u;
 }
 
 // Fill the ring from the current state: slot t = h(x); other slots 0
 // (mode 0) or h(x) (mode 1); mode 2 writes slot t only.  In the reference
 // ordering the content slot t had before (0, h(x), or the existing value)
-// becomes the stale value step t reads.
+// becomes the stale value step t reads.  One thread per instance.
 // grid (pitch / 64, N), block 64.
 extern "C" __global__ void __launch_bounds__(64)
 tvbx_history_init(const tvbx_step_args a, int mode) {
@@ -238,6 +418,7 @@ tvbx_history_init(const tvbx_step_args a, int mode) {
   const int i = blockIdx.y;
   const size_t NP = (size_t)a.n_node * a.pitch;
   const size_t ip = (size_t)i * a.pitch + p;
+  tvbx_rt *ring = (tvbx_rt *)a.ring;
   tvbx_lane_params lp;
   tvbx_load_lane(lp, a, p);
   float x[TVBX_S], y[TVBX_NOBS];
@@ -247,20 +428,25 @@ tvbx_history_init(const tvbx_step_args a, int mode) {
   const int t = a.slot0;
 #pragma unroll
   for (int r = 0; r < TVBX_R; ++r) {
-    const float v = tvbx_ring_value(r, y[r], lp, a);
-    const float z = mode == 1 ? v : tvbx_ring_value(r, 0.0f, lp, a);
+    float v = tvbx_ring_value(r, y[r], lp, a);
+    float z = mode == 1 ? v : tvbx_ring_value(r, 0.0f, lp, a);
+    if (TVBX_RING_PRE && r < TVBX_C) {
+      v = v * a.ring_scale;
+      z = z * a.ring_scale;
+    }
+    const tvbx_rt rv = tvbx_f2r(v), rz = tvbx_f2r(z);
 #if TVBX_ORDER_REF
     {
-      float *st = a.stale + (size_t)(a.step0 & 1) * TVBX_R * NP +
-                  (size_t)r * NP + ip;
-      *st = mode == 2 ? a.ring[tvbx_ring_off(t, r, i, a) + p] : z;
+      tvbx_rt *st = (tvbx_rt *)a.stale + (size_t)(a.step0 & 1) * TVBX_R * NP +
+                    (size_t)r * NP + ip;
+      *st = mode == 2 ? ring[tvbx_ring_off(t, r, i, a) + p] : rz;
     }
 #endif
     if (mode == 2) {
-      a.ring[tvbx_ring_off(t, r, i, a) + p] = v;
+      ring[tvbx_ring_off(t, r, i, a) + p] = rv;
     } else {
       for (int slot = 0; slot < a.horizon; ++slot)
-        a.ring[tvbx_ring_off(slot, r, i, a) + p] = slot == t ? v : z;
+        ring[tvbx_ring_off(slot, r, i, a) + p] = slot == t ? rv : rz;
     }
   }
 }

@@ -8,7 +8,8 @@
 // Device layouts (instance axis innermost, padded to `pitch`, a multiple of
 // 64 so one wavefront covers 64 consecutive instances):
 //   state      [S][N][pitch]            f32
-//   ring       [H][R][N][pitch]         f32   (R observables per slot)
+//   ring       [H][R][N][pitch]         f32, bf16 or f16 (R observables per
+//                                       slot; pitch a multiple of 64 * IPL)
 //   p_inst     [n_pinst][pitch]         f32   per-instance parameters
 //   p_node     [n_pnode][N]             f32   per-node parameters
 //   p_ni       [n_pni][N][pitch]        f32   per-node-per-instance parameters
@@ -53,7 +54,7 @@ typedef struct tvbx_step_args {
   const float *weight;
   /* state */
   float *state;
-  float *ring;
+  void *ring;
   /* parameters */
   const float *p_inst;
   const float *p_node;
@@ -73,11 +74,44 @@ typedef struct tvbx_step_args {
   int bold_obs;       /* observable averaged by the monitor */
   /* per-instance flag set when a state value becomes non-finite */
   unsigned *bad;
-  /* reference (loopy) ordering only: [2][R][N][pitch], by step parity, the
-   * content the slot of step n had before y(n) replaced it -- what step n
-   * reads for post_syn and for zero delays */
-  float *stale;
+  /* reference (loopy) ordering only: [2][R][N][pitch] ring elements, by
+   * step parity, the content the slot of step n had before y(n) replaced it
+   * -- what step n reads for post_syn and for zero delays */
+  void *stale;
+  /* reduced-precision rings store ring_scale * pre(y); sums are multiplied
+   * by ring_unscale (both 1 for float32 rings) */
+  float ring_scale;
+  float ring_unscale;
+  /* graph replay: when non-null, clock[0] (global step) and clock[1] (its
+   * ring slot) are added to step0 / slot0, which then hold offsets within
+   * the captured window */
+  const long long *clock;
+  /* adjoint replay: when non-null, x(n) and the coupling sums of step n are
+   * recorded at index n - rec_step0 ([W][S][N][pitch], [W][C][N][pitch]) */
+  float *rec_x;
+  float *rec_sum;
+  long long rec_step0;
 } tvbx_step_args;
+
+/* Argument block of the adjoint (reverse) kernel.  `f` carries the forward
+ * arguments (sizes, parameters, rec_x / rec_sum of the replayed window,
+ * rec_step0); step0 is the *highest* step of the launch, n_steps the number
+ * of reverse steps. */
+typedef struct tvbx_adj_args {
+  tvbx_step_args f;
+  const int *ccol;        /* [N+1] CSC column pointers (edges by source) */
+  const unsigned *crow;   /* [n_group][nnz] (delay << 16) | target */
+  const float *cw;        /* [nnz] weights in CSC order */
+  const float *rec_x;     /* [W][S][N][pitch] x(n) of the window */
+  const float *rec_sum;   /* [W][C][N][pitch] coupling sums of the window */
+  float *lam;             /* [S][N][pitch] lambda(n+1) in, lambda(n) out */
+  float *mu;              /* [H][N][pitch] ring of mu(n) */
+  const float *forcing;   /* [n_forcing][N][pitch] dL/dsample or null */
+  long long sample0;      /* global sample index of forcing[0] */
+  long long n_forcing;
+  float *g_c;             /* [n_gc][N][pitch] coupling-parameter partials */
+  float *g_m;             /* [n_gm][N][pitch] model-parameter partials */
+} tvbx_adj_args;
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,76 @@ static int fail(int code, const char *fmt, ...) {
     if (!(cond)) return fail(TVBX_E_ARG, __VA_ARGS__);                      \
   } while (0)
 
+// ------------------------------------------------------- ring element types
+// host-side conversions used when rings are uploaded / downloaded as float32
+// (round to nearest even, like the device stores)
+
+static uint16_t f32_to_bf16(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+static float bf16_to_f32(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+static uint16_t f32_to_f16(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  uint32_t mant = x & 0x7fffffu;
+  const int exp = (int)((x >> 23) & 0xffu);
+  if (exp == 0xff) return (uint16_t)(sign | 0x7c00u | (mant ? 0x200u : 0u));
+  const int e = exp - 127 + 15;
+  if (e >= 31) return (uint16_t)(sign | 0x7c00u);
+  if (e <= 0) {  // half subnormal: value = h * 2^-24
+    if (e < -10) return (uint16_t)sign;
+    mant |= 0x800000u;
+    const int shift = 14 - e;
+    uint32_t h = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u), mid = 1u << (shift - 1);
+    if (rem > mid || (rem == mid && (h & 1u))) h++;
+    return (uint16_t)(sign | h);
+  }
+  uint32_t h = ((uint32_t)e << 10) | (mant >> 13);
+  const uint32_t rem = mant & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+  return (uint16_t)(sign | h);
+}
+
+static float f16_to_f32(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  const int exp = (h >> 10) & 0x1f;
+  uint32_t mant = h & 0x3ffu, x;
+  if (exp == 0) {
+    if (!mant) {
+      x = sign;
+    } else {
+      int e = -1;
+      do {
+        ++e;
+        mant <<= 1;
+      } while (!(mant & 0x400u));
+      x = sign | ((uint32_t)(127 - 15 - e) << 23) | ((mant & 0x3ffu) << 13);
+    }
+  } else if (exp == 31) {
+    x = sign | 0x7f800000u | (mant << 13);
+  } else {
+    x = sign | ((uint32_t)(exp - 15 + 127) << 23) | (mant << 13);
+  }
+  float f;
+  memcpy(&f, &x, 4);
+  return f;
+}
+
+static size_t ring_esz(int ring_type) { return ring_type == 0 ? 4 : 2; }
+
 // ------------------------------------------------------------------ objects
 
 struct tvbx_ctx {
@@ -68,6 +138,7 @@ struct tvbx_module {
   hipModule_t mod = nullptr;
   hipFunction_t step = nullptr;
   hipFunction_t hist = nullptr;
+  hipFunction_t adj = nullptr;  // present when emitted with the adjoint
   std::vector<char> image;
 };
 
@@ -81,7 +152,8 @@ struct tvbx_sim {
   tvbx_module *mod = nullptr;
   tvbx_sim_desc d{};
   int pitch = 0;
-  size_t NP = 0;  // n_node * pitch
+  size_t NP = 0;   // n_node * pitch
+  size_t esz = 4;  // ring element size
   long long step = 0;
   int noise_mode = 0;
   unsigned long long seed = 0;
@@ -94,13 +166,39 @@ struct tvbx_sim {
   long long samples_made = 0;  // samples written since the monitor started
   long long samples_used = 0;  // samples folded into FC
   long long sample_base = 0;   // global index of the monitor's first sample
-  // timing
-  bool profile = true;
+  // timing: with `profile`, every step launch (or graph window) is bracketed
+  // by events on the simulation's stream
+  bool profile = false;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   double ms_acc = 0.0;
   long long launches = 0;
+  // hipGraph step windows: one captured window of `graph_window` steps,
+  // replayed with the step taken from the device clock
+  bool use_graph = true;
+  int graph_window = 100;
+  DevBuf clock;  // {step, step mod H}
+  bool clock_dirty = true;
+  std::vector<std::pair<long long, hipGraphExec_t>> graphs;
+  // adjoint: host copy of the CSR, its transpose on the device, checkpoints
+  std::vector<int32_t> h_row, h_col, h_delays;
+  std::vector<float> h_w;
+  DevBuf ccol, crow, cw;
+  std::vector<DevBuf> ck_state, ck_ring;
+  long long ck_step0 = -1, ck_T = 0;
+  int ck_W = 0;
+  DevBuf lam, mu, rec_x, rec_sum, forcing, g_c, g_m, g_out, save_state,
+      save_ring;
 };
+
+// advance the device clock by w steps (last node of a captured window)
+__global__ void k_tick(long long *clock, long long w, int H) {
+  if (threadIdx.x == 0) {
+    const long long s = clock[0] + w;
+    clock[0] = s;
+    clock[1] = s % H;
+  }
+}
 
 extern "C" int tvbx_abi_version(void) { return TVBX_ABI_VERSION; }
 
@@ -186,6 +284,9 @@ static int module_from_image(tvbx_ctx *c, const void *image, size_t size,
     return fail(TVBX_E_ARG, "module lacks tvbx_step / tvbx_history_init: %s",
                 hipGetErrorString(e));
   }
+  if (hipModuleGetFunction(&m->adj, m->mod, "tvbx_adjoint") != hipSuccess)
+    m->adj = nullptr;
+  (void)hipGetLastError();
   *out = m;
   return TVBX_OK;
 }
@@ -493,12 +594,14 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
   a.bold_dt = (float)(d.tavg_period * (double)d.dt * 1e-3);
   a.bold_obs = d.bold_obs;
   a.bad = (unsigned *)s->bad.p;
-  a.stale = (float *)s->stale.p;
+  a.stale = s->stale.p;
+  a.ring_scale = d.ring_type == 0 ? 1.0f : d.ring_scale;
+  a.ring_unscale = d.ring_type == 0 ? 1.0f : 1.0f / d.ring_scale;
 }
 
 // reference ordering: the stale block read by step `step`
-static float *stale_block(tvbx_sim *s, long long step) {
-  return (float *)s->stale.p + (size_t)(step & 1) * s->d.n_ring * s->NP;
+static char *stale_block(tvbx_sim *s, long long step) {
+  return (char *)s->stale.p + (size_t)(step & 1) * s->d.n_ring * s->NP * s->esz;
 }
 
 // ------------------------------------------------------------------ sim
@@ -526,17 +629,26 @@ extern "C" int tvbx_sim_create(tvbx_ctx *c, tvbx_module *m,
   CHECK(d.tavg_period == 0 || d.sample_slots >= 1, "sample_slots must be >= 1");
   CHECK(!d.fc || d.tavg_period > 0, "fc needs the monitor (tavg_period > 0)");
   CHECK(d.order == 0 || d.order == 1, "order must be 0 or 1");
+  CHECK(d.ipl == 1 || d.ipl == 2 || d.ipl == 4, "ipl must be 1, 2 or 4");
+  CHECK(d.ring_type >= 0 && d.ring_type <= 2, "ring_type must be 0, 1 or 2");
+  CHECK(d.ring_type == 0 || (std::isfinite(d.ring_scale) && d.ring_scale > 0),
+        "ring_scale must be finite and positive");
+  CHECK(d.group_size % (64 * d.ipl) == 0,
+        "group_size must be a multiple of 64 * ipl");
   HIPCHK(hipSetDevice(c->device));
   tvbx_sim *s = new tvbx_sim;
   s->ctx = c;
   s->mod = m;
   s->d = d;
-  s->pitch = (d.n_inst + 63) / 64 * 64;
+  const int tile = 64 * d.ipl;
+  s->pitch = (d.n_inst + tile - 1) / tile * tile;
+  s->esz = ring_esz(d.ring_type);
   const int pitch = s->pitch;
-  CHECK(((long long)pitch / 64) * 64 >= (long long)d.group_size * 0 + 0,
-        "pitch");
-  CHECK((long long)d.n_group * d.group_size >= pitch,
-        "n_group * group_size must cover the padded instance count");
+  if ((long long)d.n_group * d.group_size < pitch) {
+    delete s;
+    return fail(TVBX_E_ARG,
+                "n_group * group_size must cover the padded instance count");
+  }
   s->NP = (size_t)d.n_node * pitch;
   const size_t NP = s->NP, N = d.n_node;
   int rc = TVBX_OK;
@@ -546,7 +658,7 @@ extern "C" int tvbx_sim_create(tvbx_ctx *c, tvbx_module *m,
   ALLOC(coldelay, (size_t)d.n_group * std::max(d.nnz, 1) * 4);
   ALLOC(weight, (size_t)std::max(d.nnz, 1) * 4);
   ALLOC(state, (size_t)d.n_svar * NP * 4);
-  ALLOC(ring, (size_t)d.horizon * d.n_ring * NP * 4);
+  ALLOC(ring, (size_t)d.horizon * d.n_ring * NP * s->esz);
   ALLOC(p_inst, (size_t)std::max(d.n_pinst, 1) * pitch * 4);
   ALLOC(p_node, (size_t)std::max(d.n_pnode, 1) * N * 4);
   ALLOC(p_ni, (size_t)std::max(d.n_pni, 1) * NP * 4);
@@ -566,7 +678,7 @@ extern "C" int tvbx_sim_create(tvbx_ctx *c, tvbx_module *m,
     ALLOC(fc_mean, NP * 8);
     ALLOC(fc_m2, N * (N + 1) / 2 * pitch * 8);
   }
-  if (d.order == 1) ALLOC(stale, 2 * (size_t)d.n_ring * NP * 4);
+  if (d.order == 1) ALLOC(stale, 2 * (size_t)d.n_ring * NP * s->esz);
 #undef ALLOC
   if (rc != TVBX_OK) {
     tvbx_sim_destroy(s);
@@ -612,8 +724,32 @@ extern "C" int tvbx_sim_destroy(tvbx_sim *s) {
                    &s->bold_state, &s->samples, &s->bad, &s->noise,
                    &s->fc_mean, &s->fc_m2, &s->corr, &s->stale};
   for (DevBuf *b : all) dfree(*b);
+  dfree(s->clock);
+  DevBuf *adj[] = {&s->ccol, &s->crow, &s->cw, &s->lam, &s->mu, &s->rec_x,
+                   &s->rec_sum, &s->forcing, &s->g_c, &s->g_m, &s->g_out,
+                   &s->save_state, &s->save_ring};
+  for (DevBuf *b : adj) dfree(*b);
+  for (auto &b : s->ck_state) dfree(b);
+  for (auto &b : s->ck_ring) dfree(b);
   for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
+  for (auto &g : s->graphs) hipGraphExecDestroy(g.second);
   delete s;
+  return TVBX_OK;
+}
+
+static void drop_graphs(tvbx_sim *s) {
+  for (auto &g : s->graphs) hipGraphExecDestroy(g.second);
+  s->graphs.clear();
+}
+
+extern "C" int tvbx_sim_set_exec(tvbx_sim *s, int use_graph, int graph_window,
+                                 int profile) {
+  CHECK(s, "null sim");
+  CHECK(graph_window >= 1, "graph_window must be positive");
+  s->use_graph = use_graph != 0;
+  if (graph_window != s->graph_window) drop_graphs(s);
+  s->graph_window = graph_window;
+  s->profile = profile != 0;
   return TVBX_OK;
 }
 
@@ -655,6 +791,12 @@ extern "C" int tvbx_sim_set_connectome(tvbx_sim *s, const int32_t *row,
   }
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
   s->have_connectome = true;
+  // host copy for the adjoint's transpose (rebuilt on demand)
+  s->h_row.assign(row, row + d.n_node + 1);
+  s->h_col.assign(col, col + d.nnz);
+  s->h_w.assign(weight, weight + d.nnz);
+  s->h_delays.assign(delays, delays + (size_t)d.n_group * d.nnz);
+  dfree(s->ccol);
   return TVBX_OK;
 }
 
@@ -706,18 +848,45 @@ extern "C" int tvbx_sim_get_state(tvbx_sim *s, float *x) {
   return down2d(s, x, s->state.p, (size_t)s->d.n_svar * s->d.n_node);
 }
 
+// ring rows (elements of the sim's ring type) <-> host float32 [rows][P]
+static int ring_up(tvbx_sim *s, void *dst, const float *src, size_t rows) {
+  if (s->esz == 4) return up2d(s, dst, src, rows);
+  const int P = s->d.n_inst;
+  std::vector<uint16_t> tmp(rows * (size_t)P);
+  for (size_t k = 0; k < tmp.size(); ++k)
+    tmp[k] = s->d.ring_type == 1 ? f32_to_bf16(src[k]) : f32_to_f16(src[k]);
+  HIPCHK(hipMemcpy2DAsync(dst, (size_t)s->pitch * 2, tmp.data(), (size_t)P * 2,
+                          (size_t)P * 2, rows, hipMemcpyHostToDevice,
+                          s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return TVBX_OK;
+}
+
+static int ring_down(tvbx_sim *s, float *dst, const void *src, size_t rows) {
+  if (s->esz == 4) return down2d(s, dst, src, rows);
+  const int P = s->d.n_inst;
+  std::vector<uint16_t> tmp(rows * (size_t)P);
+  HIPCHK(hipMemcpy2DAsync(tmp.data(), (size_t)P * 2, src, (size_t)s->pitch * 2,
+                          (size_t)P * 2, rows, hipMemcpyDeviceToHost,
+                          s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  for (size_t k = 0; k < tmp.size(); ++k)
+    dst[k] = s->d.ring_type == 1 ? bf16_to_f32(tmp[k]) : f16_to_f32(tmp[k]);
+  return TVBX_OK;
+}
+
 extern "C" int tvbx_sim_set_ring(tvbx_sim *s, const float *r) {
   CHECK(s && r, "null argument");
   HIPCHK(hipSetDevice(s->ctx->device));
-  int rc = up2d(s, s->ring.p, r,
-                (size_t)s->d.horizon * s->d.n_ring * s->d.n_node);
+  int rc = ring_up(s, s->ring.p, r,
+                   (size_t)s->d.horizon * s->d.n_ring * s->d.n_node);
   if (rc != TVBX_OK || s->d.order != 1) return rc;
   // reference ordering: the current slot's content is what the next step
   // reads for post_syn and zero delays
   const size_t slot = (size_t)(s->step % s->d.horizon);
-  const size_t blk = (size_t)s->d.n_ring * s->NP;
+  const size_t blk = (size_t)s->d.n_ring * s->NP * s->esz;
   HIPCHK(hipMemcpyAsync(stale_block(s, s->step),
-                        (const float *)s->ring.p + slot * blk, blk * 4,
+                        (const char *)s->ring.p + slot * blk, blk,
                         hipMemcpyDeviceToDevice, s->ctx->stream));
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
   return TVBX_OK;
@@ -727,13 +896,13 @@ extern "C" int tvbx_sim_get_ring(tvbx_sim *s, float *r) {
   CHECK(s && r, "null argument");
   HIPCHK(hipSetDevice(s->ctx->device));
   const tvbx_sim_desc &d = s->d;
-  int rc = down2d(s, r, s->ring.p, (size_t)d.horizon * d.n_ring * d.n_node);
+  int rc = ring_down(s, r, s->ring.p, (size_t)d.horizon * d.n_ring * d.n_node);
   if (rc != TVBX_OK || d.order != 1) return rc;
   // reference ordering: slot (step mod H) has been written ahead of time;
   // report its not-yet-rewritten content, as the reference's obsrv holds
   const size_t slot = (size_t)(s->step % d.horizon);
-  return down2d(s, r + slot * d.n_ring * d.n_node * (size_t)d.n_inst,
-                stale_block(s, s->step), (size_t)d.n_ring * d.n_node);
+  return ring_down(s, r + slot * d.n_ring * d.n_node * (size_t)d.n_inst,
+                   stale_block(s, s->step), (size_t)d.n_ring * d.n_node);
 }
 
 extern "C" int tvbx_sim_set_step(tvbx_sim *s, long long step) {
@@ -747,6 +916,7 @@ extern "C" int tvbx_sim_set_step(tvbx_sim *s, long long step) {
     s->samples_used = 0;
   }
   s->step = step;
+  s->clock_dirty = true;
   return TVBX_OK;
 }
 
@@ -771,6 +941,7 @@ extern "C" int tvbx_sim_init_history(tvbx_sim *s, int mode) {
 extern "C" int tvbx_sim_set_noise(tvbx_sim *s, int mode,
                                   unsigned long long seed) {
   CHECK(s && (mode == 0 || mode == 1), "bad argument");
+  if (mode != s->noise_mode || seed != s->seed) drop_graphs(s);
   s->noise_mode = mode;
   s->seed = seed;
   return TVBX_OK;
@@ -819,6 +990,104 @@ static int collect_timing(tvbx_sim *s) {
   return TVBX_OK;
 }
 
+static long long gcd_ll(long long a, long long b) {
+  while (b) {
+    long long t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// length of a graph window: graph_window rounded to a multiple of the steps
+// per launch and of the monitor period; 0 when graphs are off
+static long long graph_len(tvbx_sim *s) {
+  if (!s->use_graph) return 0;
+  long long L = s->d.steps_per_launch;
+  if (s->d.tavg_period > 0)
+    L = L / gcd_ll(L, s->d.tavg_period) * s->d.tavg_period;
+  long long W = s->graph_window - s->graph_window % L;
+  if (W < L) W = L;
+  return W;
+}
+
+// capture one window of W steps: W / steps_per_launch step launches whose
+// step / slot are offsets from the device clock, then the clock tick
+static int capture_window(tvbx_sim *s, long long W, hipGraphExec_t *out) {
+  const tvbx_sim_desc &d = s->d;
+  const unsigned gx = s->pitch / (64 * d.ipl);
+  const unsigned gy = (d.n_node + d.node_block - 1) / d.node_block;
+  const unsigned bx = 64u * d.waves;
+  hipStream_t st = s->ctx->stream;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  for (long long k = 0; k < W && e == hipSuccess; k += d.steps_per_launch) {
+    tvbx_step_args a;
+    fill_args(s, a);
+    a.step0 = k;
+    a.slot0 = (int)(k % d.horizon);
+    a.n_steps = d.steps_per_launch;
+    a.clock = (const long long *)s->clock.p;
+    void *params[] = {&a};
+    e = hipModuleLaunchKernel(s->mod->step, gx, gy, 1, bx, 1, 1, 0, st,
+                              params, nullptr);
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st,
+                       (long long *)s->clock.p, W, d.horizon);
+    e = hipGetLastError();
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(st, &g);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  if (g) hipGraphDestroy(g);
+  if (e != hipSuccess)
+    return fail(TVBX_E_HIP, "graph capture of %lld steps: %s", W,
+                hipGetErrorString(e));
+  return TVBX_OK;
+}
+
+static int run_graph(tvbx_sim *s, long long W) {
+  const tvbx_sim_desc &d = s->d;
+  if (!s->clock.p) {
+    int rc = dalloc(s->clock, 2 * sizeof(long long));
+    if (rc != TVBX_OK) return rc;
+    s->clock_dirty = true;
+  }
+  if (s->clock_dirty) {
+    long long h[2] = {s->step, s->step % d.horizon};
+    HIPCHK(hipMemcpyAsync(s->clock.p, h, sizeof h, hipMemcpyHostToDevice,
+                          s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    s->clock_dirty = false;
+  }
+  hipGraphExec_t ex = nullptr;
+  for (auto &g : s->graphs)
+    if (g.first == W) ex = g.second;
+  if (!ex) {
+    int rc = capture_window(s, W, &ex);
+    if (rc != TVBX_OK) return rc;
+    s->graphs.push_back({W, ex});
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (s->profile) {
+    e0 = next_event(s);
+    e1 = next_event(s);
+    if (e0) hipEventRecord(e0, s->ctx->stream);
+  }
+  HIPCHK(hipGraphLaunch(ex, s->ctx->stream));
+  if (s->profile && e1) hipEventRecord(e1, s->ctx->stream);
+  s->launches += W / d.steps_per_launch;
+  if (d.tavg_period > 0) {
+    const long long K = d.tavg_period;
+    s->samples_made += (s->step + W) / K - s->step / K;
+  }
+  s->step += W;
+  if (s->ev_used >= 4096) return collect_timing(s);
+  return TVBX_OK;
+}
+
 extern "C" int tvbx_sim_run(tvbx_sim *s, long long n_steps,
                             const float *noise, long long tstride) {
   CHECK(s && n_steps >= 0, "bad argument");
@@ -836,11 +1105,30 @@ extern "C" int tvbx_sim_run(tvbx_sim *s, long long n_steps,
     rc = up2d(s, s->noise.p, noise, steps_up * d.n_svar * d.n_node);
     if (rc != TVBX_OK) return rc;
   }
-  const unsigned gx = s->pitch / 64;
+  const unsigned gx = s->pitch / (64 * d.ipl);
   const unsigned gy = (d.n_node + d.node_block - 1) / d.node_block;
   const unsigned bx = 64u * d.waves;
+  const long long W = noise ? 0 : graph_len(s);
   long long done = 0;
   while (done < n_steps) {
+    if (W > 0 && n_steps - done >= W) {
+      // a whole captured window, if its samples fit the monitor buffer
+      bool fits = true;
+      if (d.tavg_period > 0) {
+        const long long ns = W / d.tavg_period;
+        if (d.fc && d.sample_slots - (s->samples_made - s->samples_used) < ns) {
+          int rc = fc_flush(s);
+          if (rc != TVBX_OK) return rc;
+        }
+        fits = ns <= d.sample_slots;
+      }
+      if (fits) {
+        int rc = run_graph(s, W);
+        if (rc != TVBX_OK) return rc;
+        done += W;
+        continue;
+      }
+    }
     long long chunk = std::min<long long>(n_steps - done, d.steps_per_launch);
     if (d.tavg_period > 0) {
       // do not produce more samples than the buffer holds unconsumed
@@ -881,6 +1169,7 @@ extern "C" int tvbx_sim_run(tvbx_sim *s, long long n_steps,
       s->samples_made += (s->step + chunk) / K - s->step / K;
     }
     s->step += chunk;
+    s->clock_dirty = true;  // eager launches advance the host step only
     done += chunk;
     if (s->ev_used >= 4096) {
       int rc = collect_timing(s);
@@ -1059,4 +1348,240 @@ extern "C" int tvbx_corr(tvbx_ctx *c, const float *samples, long long n,
   dfree(s.fc_m2);
   dfree(s.corr);
   return rc;
+}
+
+// ------------------------------------------------------------------ adjoint
+
+// sum_j src[g][j][p] -> dst[g][p] (sequential over j: deterministic)
+__global__ void k_reduce_nodes(const float *__restrict__ src, int N, int pitch,
+                               float *__restrict__ dst) {
+  const int p = blockIdx.x * 64 + threadIdx.x;
+  const int g = blockIdx.y;
+  const float *s = src + (size_t)g * N * pitch + p;
+  float acc = 0.0f;
+  for (int j = 0; j < N; ++j) acc += s[(size_t)j * pitch];
+  dst[(size_t)g * pitch + p] = acc;
+}
+
+static int adjoint_checks(tvbx_sim *s) {
+  const tvbx_sim_desc &d = s->d;
+  if (!s->mod->adj)
+    return fail(TVBX_E_STATE, "module was emitted without the adjoint");
+  CHECK(d.order == 0, "the adjoint follows order 0 (cuda) only");
+  CHECK(d.ring_type == 0, "the adjoint needs a float32 history");
+  CHECK(!d.bold, "the adjoint does not cover the BOLD stage");
+  CHECK(d.n_coupled == 1, "the adjoint needs one coupled observable");
+  return TVBX_OK;
+}
+
+// transpose of the CSR per delay group: edges grouped by source, in the
+// order targets are visited by the forward rows
+static int build_csc(tvbx_sim *s) {
+  if (s->ccol.p) return TVBX_OK;
+  const tvbx_sim_desc &d = s->d;
+  const int N = d.n_node, nnz = d.nnz;
+  std::vector<int32_t> cnt(N + 1, 0);
+  for (int e = 0; e < nnz; ++e) cnt[s->h_col[e] + 1]++;
+  for (int j = 0; j < N; ++j) cnt[j + 1] += cnt[j];
+  std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1), perm(nnz);
+  std::vector<int32_t> tgt(nnz);
+  for (int i = 0; i < N; ++i)
+    for (int e = s->h_row[i]; e < s->h_row[i + 1]; ++e) {
+      const int k = pos[s->h_col[e]]++;
+      perm[k] = e;
+      tgt[k] = i;
+    }
+  std::vector<unsigned> crow((size_t)d.n_group * std::max(nnz, 1), 0u);
+  std::vector<float> cw(std::max(nnz, 1), 0.0f);
+  for (int k = 0; k < nnz; ++k) cw[k] = s->h_w[perm[k]];
+  for (int g = 0; g < d.n_group; ++g)
+    for (int k = 0; k < nnz; ++k) {
+      const int dl = s->h_delays[(size_t)g * nnz + perm[k]];
+      CHECK(dl >= 1, "the adjoint needs delays >= 1 (edge %d has 0)", perm[k]);
+      crow[(size_t)g * nnz + k] = ((unsigned)dl << 16) | (unsigned)tgt[k];
+    }
+  int rc = dalloc(s->ccol, (size_t)(N + 1) * 4);
+  if (rc == TVBX_OK) rc = dalloc(s->crow, crow.size() * 4);
+  if (rc == TVBX_OK) rc = dalloc(s->cw, cw.size() * 4);
+  if (rc != TVBX_OK) return rc;
+  HIPCHK(hipMemcpyAsync(s->ccol.p, cnt.data(), (size_t)(N + 1) * 4,
+                        hipMemcpyHostToDevice, s->ctx->stream));
+  HIPCHK(hipMemcpyAsync(s->crow.p, crow.data(), crow.size() * 4,
+                        hipMemcpyHostToDevice, s->ctx->stream));
+  HIPCHK(hipMemcpyAsync(s->cw.p, cw.data(), cw.size() * 4,
+                        hipMemcpyHostToDevice, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_sim_checkpoint_run(tvbx_sim *s, long long n_steps,
+                                       int window) {
+  CHECK(s && n_steps > 0 && window > 0, "bad argument");
+  CHECK(n_steps % window == 0, "n_steps must be a multiple of window");
+  int rc = adjoint_checks(s);
+  if (rc != TVBX_OK) return rc;
+  const tvbx_sim_desc &d = s->d;
+  HIPCHK(hipSetDevice(s->ctx->device));
+  const long long K = n_steps / window;
+  const size_t sb = s->state.bytes, rb = s->ring.bytes;
+  if ((long long)s->ck_state.size() != K) {
+    for (auto &b : s->ck_state) dfree(b);
+    for (auto &b : s->ck_ring) dfree(b);
+    s->ck_state.assign(K, DevBuf());
+    s->ck_ring.assign(K, DevBuf());
+  }
+  for (long long k = 0; k < K; ++k) {
+    rc = dalloc(s->ck_state[k], sb);
+    if (rc == TVBX_OK) rc = dalloc(s->ck_ring[k], rb);
+    if (rc != TVBX_OK) return rc;
+  }
+  s->ck_step0 = s->step;
+  s->ck_T = n_steps;
+  s->ck_W = window;
+  for (long long k = 0; k < K; ++k) {
+    HIPCHK(hipMemcpyAsync(s->ck_state[k].p, s->state.p, sb,
+                          hipMemcpyDeviceToDevice, s->ctx->stream));
+    HIPCHK(hipMemcpyAsync(s->ck_ring[k].p, s->ring.p, rb,
+                          hipMemcpyDeviceToDevice, s->ctx->stream));
+    rc = tvbx_sim_run(s, window, nullptr, 0);
+    if (rc != TVBX_OK) return rc;
+  }
+  (void)d;
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
+                                int on_device, long long sample0,
+                                long long n_samples, int n_gc,
+                                float *g_coupling, int n_gm, float *g_model,
+                                int out_on_device) {
+  CHECK(s, "null sim");
+  int rc = adjoint_checks(s);
+  if (rc != TVBX_OK) return rc;
+  CHECK(s->ck_T > 0, "run tvbx_sim_checkpoint_run first");
+  CHECK(n_samples >= 0 && (n_samples == 0 || dsamples), "bad forcing");
+  const tvbx_sim_desc &d = s->d;
+  HIPCHK(hipSetDevice(s->ctx->device));
+  rc = build_csc(s);
+  if (rc != TVBX_OK) return rc;
+  const int N = d.n_node, S = d.n_svar, P = d.n_inst, pitch = s->pitch;
+  const size_t NP = s->NP;
+  const long long W = s->ck_W, K = s->ck_T / W;
+  CHECK(n_gc >= 0 && n_gc <= d.n_pinst, "n_gc exceeds the instance parameters");
+  CHECK(n_gm >= 0 && n_gm <= d.n_pinst + d.n_pni,
+        "n_gm exceeds the instance / node-instance parameters");
+  const int ngc = std::max(n_gc, 1), ngm = std::max(n_gm, 1);
+  int r2 = TVBX_OK;
+#define AL(buf, bytes) \
+  if (r2 == TVBX_OK) r2 = dalloc(s->buf, (size_t)(bytes));
+  AL(lam, (size_t)S * NP * 4);
+  AL(mu, (size_t)d.horizon * NP * 4);
+  AL(rec_x, (size_t)W * S * NP * 4);
+  AL(rec_sum, (size_t)W * NP * 4);
+  AL(forcing, (size_t)std::max<long long>(n_samples, 1) * NP * 4);
+  AL(g_c, (size_t)ngc * NP * 4);
+  AL(g_m, (size_t)ngm * NP * 4);
+  AL(g_out, (size_t)ngc * pitch * 4);
+  AL(save_state, s->state.bytes);
+  AL(save_ring, s->ring.bytes);
+#undef AL
+  if (r2 != TVBX_OK) return r2;
+  DevBuf *z[] = {&s->lam, &s->mu, &s->forcing, &s->g_c, &s->g_m};
+  for (DevBuf *b : z) {
+    rc = fill_f32(s, *b, 0.0f);
+    if (rc != TVBX_OK) return rc;
+  }
+  if (n_samples) {
+    HIPCHK(hipMemcpy2DAsync(s->forcing.p, (size_t)pitch * 4, dsamples,
+                            (size_t)P * 4, (size_t)P * 4,
+                            (size_t)n_samples * N,
+                            on_device ? hipMemcpyDeviceToDevice
+                                      : hipMemcpyHostToDevice,
+                            s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  }
+  // keep the end-of-run state so the simulation can continue afterwards
+  const long long step_end = s->step;
+  HIPCHK(hipMemcpyAsync(s->save_state.p, s->state.p, s->state.bytes,
+                        hipMemcpyDeviceToDevice, s->ctx->stream));
+  HIPCHK(hipMemcpyAsync(s->save_ring.p, s->ring.p, s->ring.bytes,
+                        hipMemcpyDeviceToDevice, s->ctx->stream));
+  const unsigned gx = pitch / (64 * d.ipl);
+  const unsigned gy = (N + d.node_block - 1) / d.node_block;
+  const unsigned bx = 64u * d.waves;
+  const int spl = d.steps_per_launch;
+  for (long long kw = K - 1; kw >= 0; --kw) {
+    HIPCHK(hipMemcpyAsync(s->state.p, s->ck_state[kw].p, s->state.bytes,
+                          hipMemcpyDeviceToDevice, s->ctx->stream));
+    HIPCHK(hipMemcpyAsync(s->ring.p, s->ck_ring[kw].p, s->ring.bytes,
+                          hipMemcpyDeviceToDevice, s->ctx->stream));
+    const long long n_lo = s->ck_step0 + kw * W;
+    // forward replay of the window, recording x(n) and sum(n)
+    for (long long k = 0; k < W; k += spl) {
+      tvbx_step_args a;
+      s->step = n_lo + k;
+      fill_args(s, a);
+      a.n_steps = (int)std::min<long long>(spl, W - k);
+      a.tavg_period = 0;
+      a.out_input = nullptr;
+      a.rec_x = (float *)s->rec_x.p;
+      a.rec_sum = (float *)s->rec_sum.p;
+      a.rec_step0 = n_lo;
+      void *params[] = {&a};
+      HIPCHK(hipModuleLaunchKernel(s->mod->step, gx, gy, 1, bx, 1, 1, 0,
+                                   s->ctx->stream, params, nullptr));
+    }
+    // reverse sweep of the window
+    for (long long k = 0; k < W; k += spl) {
+      tvbx_adj_args A;
+      memset(&A, 0, sizeof A);
+      s->step = n_lo;
+      fill_args(s, A.f);
+      A.f.step0 = n_lo + W - 1 - k;
+      A.f.n_steps = (int)std::min<long long>(spl, W - k);
+      A.f.rec_step0 = n_lo;
+      A.ccol = (const int *)s->ccol.p;
+      A.crow = (const unsigned *)s->crow.p;
+      A.cw = (const float *)s->cw.p;
+      A.rec_x = (const float *)s->rec_x.p;
+      A.rec_sum = (const float *)s->rec_sum.p;
+      A.lam = (float *)s->lam.p;
+      A.mu = (float *)s->mu.p;
+      A.forcing = n_samples ? (const float *)s->forcing.p : nullptr;
+      A.sample0 = sample0;
+      A.n_forcing = n_samples;
+      A.g_c = (float *)s->g_c.p;
+      A.g_m = (float *)s->g_m.p;
+      void *params[] = {&A};
+      HIPCHK(hipModuleLaunchKernel(s->mod->adj, gx, gy, 1, bx, 1, 1, 0,
+                                   s->ctx->stream, params, nullptr));
+    }
+  }
+  // restore the end-of-run state
+  HIPCHK(hipMemcpyAsync(s->state.p, s->save_state.p, s->state.bytes,
+                        hipMemcpyDeviceToDevice, s->ctx->stream));
+  HIPCHK(hipMemcpyAsync(s->ring.p, s->save_ring.p, s->ring.bytes,
+                        hipMemcpyDeviceToDevice, s->ctx->stream));
+  s->step = step_end;
+  s->clock_dirty = true;
+  // outputs: coupling gradients summed over nodes [n_gc][P]; model
+  // gradients per node [n_gm][N][P]
+  const hipMemcpyKind ok = out_on_device ? hipMemcpyDeviceToDevice
+                                         : hipMemcpyDeviceToHost;
+  if (g_coupling && n_gc) {
+    hipLaunchKernelGGL(k_reduce_nodes, dim3(pitch / 64, n_gc), dim3(64),
+                       0, s->ctx->stream, (const float *)s->g_c.p, N, pitch,
+                       (float *)s->g_out.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy2DAsync(g_coupling, (size_t)P * 4, s->g_out.p,
+                            (size_t)pitch * 4, (size_t)P * 4, n_gc, ok,
+                            s->ctx->stream));
+  }
+  if (g_model && n_gm) {
+    HIPCHK(hipMemcpy2DAsync(g_model, (size_t)P * 4, s->g_m.p,
+                            (size_t)pitch * 4, (size_t)P * 4,
+                            (size_t)n_gm * N, ok, s->ctx->stream));
+  }
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return TVBX_OK;
 }

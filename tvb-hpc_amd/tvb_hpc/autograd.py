@@ -1,0 +1,80 @@
+"""
+PyTorch autograd wrapper of the device integrator's hand-written adjoint.
+
+``integrate(sim, params, x0, n_steps, window)`` runs the simulation forward
+(keeping a checkpoint every ``window`` steps) and returns its monitor samples
+(n_samples, N, P) as a tensor; backward replays each window from its
+checkpoint and runs the reverse sweep of the delayed Euler-Maruyama
+recursion (csrc/adjoint_template.hip), returning dL/dparam for the tensors in
+``params`` (coupling parameters (P,), node-and-instance model parameters
+(N, P)).  Any differentiable torch loss of the samples can sit on top, e.g.
+``fc_loss`` below (1 - Pearson correlation of simulated and target FC, the
+Bayesian-inversion use the reference names in README.md:3-4).
+
+PyTorch is used only for the autograd plumbing and the loss; the forward and
+reverse sweeps are the HIP kernels behind libtvbx.
+"""
+
+import numpy as np
+import torch
+
+
+class _Integrate(torch.autograd.Function):
+
+    @staticmethod
+    def forward(ctx, sim, keys, x0, n_steps, window, seed, *values):
+        params = {k: v.detach().double().cpu().numpy()
+                  for k, v in zip(keys, values)}
+        sim.set_params(params)
+        sim.set_state(x0)
+        sim.step = 0
+        sim.init_history('zeros')
+        sim.set_noise(seed)
+        sim.checkpoint_run(n_steps, window)
+        samples, first = sim.samples()
+        ctx.sim, ctx.keys, ctx.first = sim, keys, first
+        ctx.device = values[0].device
+        ctx.dtypes = [v.dtype for v in values]
+        return torch.from_numpy(samples).to(ctx.device)
+
+    @staticmethod
+    def backward(ctx, d_samples):
+        g = ctx.sim.adjoint(d_samples.detach().float().cpu().numpy(),
+                            ctx.first)
+        grads = [torch.from_numpy(np.asarray(g[k])).to(ctx.device, dt)
+                 for k, dt in zip(ctx.keys, ctx.dtypes)]
+        return (None, None, None, None, None, None) + tuple(grads)
+
+
+def integrate(sim, params, x0, n_steps, window, seed=None):
+    """Monitor samples (n_steps // tavg_period, N, P) of ``sim`` from state
+    x0 (S, N, P), differentiable w.r.t. the tensors in ``params``
+    ({'c:a': (P,), 'm:eta': (N, P), ...}; keys as in ``Simulation``)."""
+    keys = list(params)
+    return _Integrate.apply(sim, keys, x0, int(n_steps), int(window), seed,
+                            *[params[k] for k in keys])
+
+
+def fc(samples):
+    """Pearson FC per instance: samples (M, N, P) -> (P, N, N)."""
+    x = samples - samples.mean(dim=0, keepdim=True)
+    cov = torch.einsum('mip,mjp->pij', x, x)
+    d = torch.sqrt(torch.diagonal(cov, dim1=1, dim2=2))
+    return cov / (d[:, :, None] * d[:, None, :])
+
+
+def fc_loss(samples, fc_target):
+    """Per-instance 1 - corr(upper triangle of FC(samples), of fc_target);
+    samples (M, N, P), fc_target (N, N) or (P, N, N) -> (P,)."""
+    f = fc(samples.double())
+    N = f.shape[-1]
+    iu = torch.triu_indices(N, N, 1)
+    a = f[:, iu[0], iu[1]]
+    b = torch.as_tensor(fc_target, dtype=a.dtype, device=a.device)
+    b = b[..., iu[0], iu[1]]
+    if b.ndim == 1:
+        b = b.expand_as(a)
+    a = a - a.mean(dim=1, keepdim=True)
+    b = b - b.mean(dim=1, keepdim=True)
+    r = (a * b).sum(dim=1) / torch.sqrt((a * a).sum(dim=1) * (b * b).sum(dim=1))
+    return 1.0 - r

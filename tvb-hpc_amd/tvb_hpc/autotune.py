@@ -1,0 +1,126 @@
+"""
+Launch-shape autotuner for the step kernels.
+
+The choices:
+
+* ``ipl``: sweep instances per lane (1, 2, 4).  A wavefront covers
+  64 * ipl consecutive instances, so each delayed-observable gather is one
+  contiguous 256 B .. 1 KiB read (float32 ring) -- longer reads use HBM
+  better -- at the cost of ipl-fold registers per lane and fewer
+  workgroups.  Delay groups must be multiples of 64 * ipl instances.
+* ``node_block`` / ``waves``: nodes per workgroup and waves per workgroup.
+  Each wave takes one node at a time.  With node_block < N the node axis is
+  split over workgroups and every step is one launch (grid-wide dependency
+  between steps); with node_block >= N one workgroup holds a whole instance
+  tile and ``steps_per_launch`` steps run inside one launch, separated by
+  workgroup barriers.
+
+``tune(cfg)`` times candidates on the device with short runs and records
+the best per (kernel source, N, P, nnz, device) in a JSON table next to the
+library; ``best_launch(cfg)`` reads that table, falling back to measured
+defaults for MI355X.
+"""
+
+import json
+import os
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, '_lib', 'tuned.json')
+
+
+def valid_ipls(cfg):
+    P = cfg.n_inst
+    out = []
+    for ipl in (1, 2, 4):
+        tile = 64 * ipl
+        if cfg.group_size is not None and cfg.group_size % tile:
+            continue
+        # keep at least a few instance tiles per node block
+        if ipl > 1 and P < 2 * tile:
+            continue
+        out.append(ipl)
+    return out
+
+
+def candidates(cfg):
+    N = cfg.csr.nnode
+    out = []
+    for ipl in valid_ipls(cfg):
+        for waves, nb in ((4, 4), (2, 2), (8, 8), (4, 8), (8, 16), (1, 1),
+                          (16, 16)):
+            if nb <= N:
+                out.append(dict(ipl=ipl, waves=waves, node_block=nb,
+                                steps_per_launch=1))
+        if N <= 400:
+            for waves in (8, 16):
+                out.append(dict(ipl=ipl, waves=waves, node_block=N,
+                                steps_per_launch=50))
+    return out
+
+
+def default_launch(cfg):
+    """Measured defaults: per-step launches with 4-node workgroups, the
+    widest ipl the instance count allows."""
+    ipls = valid_ipls(cfg)
+    ipl = 4 if 4 in ipls and cfg.n_inst >= 1024 else max(
+        [i for i in ipls if i <= 2] or [1])
+    return dict(ipl=ipl, waves=4, node_block=4, steps_per_launch=1)
+
+
+def _key(cfg, ctx_info):
+    return '%s|N%d|P%d|nnz%d|%s' % (cfg.spec().key(), cfg.csr.nnode,
+                                    cfg.n_inst, cfg.csr.nnz,
+                                    ctx_info.get('arch', '?'))
+
+
+def _load():
+    try:
+        with open(TABLE) as fd:
+            return json.load(fd)
+    except (OSError, ValueError):
+        return {}
+
+
+def best_launch(cfg, ctx=None):
+    table = _load()
+    if table:
+        from .hip import Context
+        ctx = ctx or Context.default()
+        hit = table.get(_key(cfg, ctx.info()))
+        if hit:
+            return hit['launch']
+    return default_launch(cfg)
+
+
+def tune(cfg, steps=100, warmup=10, ctx=None, log=None):
+    """Time every candidate for `steps` steps; record and return the best."""
+    from .hip import Context
+    ctx = ctx or Context.default()
+    x0 = cfg.initial_state()
+    results = []
+    for cand in candidates(cfg):
+        sim = cfg.simulation(context=ctx, **cand)
+        sim.set_state(x0)
+        sim.init_history('zeros')
+        sim.run(warmup)
+        sim.sync()
+        t = time.perf_counter()
+        sim.run(steps)
+        sim.sync()
+        ms = (time.perf_counter() - t) / steps * 1e3
+        results.append((ms, cand))
+        if log:
+            log('%s %.4f ms/step' % (cand, ms))
+        del sim
+    results.sort(key=lambda r: r[0])
+    best = results[0][1]
+    table = _load()
+    table[_key(cfg, ctx.info())] = {'launch': best, 'ms_per_step':
+                                    results[0][0], 'tried': len(results)}
+    os.makedirs(os.path.dirname(TABLE), exist_ok=True)
+    tmp = TABLE + '.%d.tmp' % os.getpid()
+    with open(tmp, 'w') as fd:
+        json.dump(table, fd, indent=1, sort_keys=True)
+    os.replace(tmp, TABLE)
+    return best, results

@@ -78,6 +78,12 @@ def build_all(jobs=8, kernels=True):
     if not kernels:
         return
     specs = builtin_specs()
+    # drop code objects of sources that no longer exist
+    keep = {s.key() + '.hsaco' for s in specs}
+    if os.path.isdir(KDIR):
+        for f in os.listdir(KDIR):
+            if f.endswith('.hsaco') and f not in keep:
+                os.remove(os.path.join(KDIR, f))
     from concurrent.futures import ThreadPoolExecutor
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(build_kernel, specs))

@@ -47,6 +47,9 @@ class SweepConfig:
     bold_obs: int = 0
     seed: int = 0
     description: str = ''
+    ring_dtype: str = 'f32'
+    ring_scale: float = 1.0
+    delay_bins: Optional[int] = None
 
     @property
     def horizon(self):
@@ -73,7 +76,8 @@ class SweepConfig:
                            csr, n_inst, self.n_steps, params,
                            dict(self.kinds), gs, self.tavg_period, self.bold,
                            self.fc, self.bold_obs, self.seed,
-                           self.description)
+                           self.description, self.ring_dtype, self.ring_scale,
+                           self.delay_bins)
 
     def initial_state(self, dtype=np.float32):
         rng = np.random.default_rng(1000 + self.seed)
@@ -84,7 +88,8 @@ class SweepConfig:
         from .hip import Simulation
         opts = dict(params=self.params, kinds=self.kinds,
                     group_size=self.group_size, tavg_period=self.tavg_period,
-                    bold=self.bold, fc=self.fc, bold_obs=self.bold_obs)
+                    bold=self.bold, fc=self.fc, bold_obs=self.bold_obs,
+                    ring_dtype=self.ring_dtype, ring_scale=self.ring_scale)
         opts.update(kw)
         return Simulation(self.model, self.cfun, self.scheme, self.csr,
                           self.n_inst, **opts)
@@ -103,6 +108,7 @@ class SweepConfig:
         sk = kinds.pop('sigma', None)
         if sk is None and getattr(self.scheme, 'sigma', None) is not None:
             sk = 'const'
+        kw.setdefault('ring_dtype', self.ring_dtype)
         return KernelSpec(self.model, self.cfun, self.scheme, kinds=kinds,
                           sigma_kind=sk, **kw)
 
@@ -167,6 +173,22 @@ def jansenrit998(n_inst=1024, n_steps=10000, tavg_period=20):
         description='Jansen-Rit N=998, P=%d (G x mu), dt 0.05 ms' % n_inst)
 
 
+def jansenrit998_mixed(n_inst=1024, n_steps=10000, tavg_period=20,
+                       ring_dtype='bf16', bins=32):
+    """Jansen-Rit N=998 with delays quantised to `bins` values and a
+    reduced-precision history ring (bf16 / f16) holding 1/cmax-scaled
+    sigmoid outputs; state, sums and FC stay float32 / float64."""
+    cfg = jansenrit998(n_inst, n_steps, tavg_period)
+    csr, _ = cfg.csr.binned(bins)
+    cfg.csr = csr.sorted_by_delay()
+    cfg.name = 'jansenrit998_%s_b%d' % (ring_dtype, bins)
+    cfg.ring_dtype = ring_dtype
+    cfg.ring_scale = 1.0 / float(cfg.cfun.param['cmax'])
+    cfg.delay_bins = bins
+    cfg.description += ', %d delay bins, %s ring' % (bins, ring_dtype)
+    return cfg
+
+
 def mpr84(n_inst=4096, n_steps=20000, G=None, tavg_period=10):
     "Differentiation for inversion: MPR N=84."
     N, dt, v = 84, 0.1, 1.0
@@ -190,13 +212,20 @@ def hcp0_kuramoto(path):
 
 
 ALL = {'kuramoto76': kuramoto76, 'g2do84': g2do84,
-       'jansenrit998': jansenrit998, 'mpr84': mpr84}
+       'jansenrit998': jansenrit998, 'mpr84': mpr84,
+       'jansenrit998_bf16': lambda **kw: jansenrit998_mixed(
+           ring_dtype='bf16', **kw),
+       'jansenrit998_f16': lambda **kw: jansenrit998_mixed(
+           ring_dtype='f16', **kw)}
 
 
 def all_specs():
-    "Kernel specs the builtin configurations use (for ahead-of-time builds)."
+    """Kernel specs the builtin configurations use, for every launch width
+    the autotuner may pick (for ahead-of-time builds)."""
+    from .autotune import valid_ipls
     specs = []
     for fn in ALL.values():
         cfg = fn()
-        specs.append(cfg.spec())
+        for ipl in valid_ipls(cfg):
+            specs.append(cfg.spec(ipl=ipl))
     return specs

@@ -42,6 +42,8 @@ from .scheme import EulerMaryuyamaStep, EulerStep
 
 CSRC = os.path.normpath(os.path.join(os.path.dirname(__file__), '..', 'csrc'))
 KINDS = ('const', 'glob', 'inst', 'node', 'ni')
+# ring element types: float32, bfloat16, float16
+RING_TYPES = {'f32': 0, 'bf16': 1, 'f16': 2}
 
 # hipRTC / hipcc options shared by the run-time and ahead-of-time builds:
 # no contraction (the only fused multiply-add is the explicit fmaf of the
@@ -92,11 +94,18 @@ class KernelSpec:
 
     def __init__(self, model: BaseModel, cfun: BaseCoupling, scheme: EulerStep,
                  kinds=None, sigma_kind=None, ring='coupled', ring_pre=None,
-                 order='cuda'):
+                 order='cuda', ipl=1, ring_dtype='f32', adjoint=False):
         self.model, self.cfun, self.scheme = model, cfun, scheme
+        self.adjoint = bool(adjoint)
         if order not in ('cuda', 'reference'):
             raise ValueError('order must be cuda or reference')
         self.order = order
+        if ipl not in (1, 2, 4):
+            raise ValueError('ipl must be 1, 2 or 4')
+        self.ipl = int(ipl)
+        if ring_dtype not in RING_TYPES:
+            raise ValueError('ring_dtype must be one of %s' % list(RING_TYPES))
+        self.ring_dtype = ring_dtype
         self.S = len(model.state_sym)
         self.C = len(cfun.io)
         self.NOBS = len(model.obsrv_sym)
@@ -237,6 +246,8 @@ class KernelSpec:
         lines.append('#define TVBX_USES_POST %d' % int(self.uses_post))
         lines.append('#define TVBX_ORDER_REF %d' % int(self.order ==
                                                      'reference'))
+        lines.append('#define TVBX_IPL %d' % self.ipl)
+        lines.append('#define TVBX_RING_T %d' % RING_TYPES[self.ring_dtype])
         # lane parameters
         fields = self._lane_fields()
         lines.append('struct tvbx_lane_params {')
@@ -366,13 +377,146 @@ class KernelSpec:
         out.append('}')
         return out
 
+    # -- adjoint ---------------------------------------------------------------
+
+    def grad_params(self):
+        """Parameters the adjoint differentiates: coupling parameters of kind
+        'inst' (e.g. the global coupling G) and model parameters of kind
+        'inst' or 'ni' (e.g. a per-node excitability)."""
+        gc = [k for k in sorted(self.kinds) if k.startswith('c:')
+              and self.kinds[k] == 'inst']
+        gm = [k for k in sorted(self.kinds) if k.startswith('m:')
+              and self.kinds[k] in ('inst', 'ni')]
+        return gc, gm
+
+    def _drift_inlined(self):
+        "drift expressions with the auxiliary variables substituted"
+        aux = {}
+        for lhs, rhs in self.model.auxex_exprs():
+            aux[lhs] = E.subst(rhs, aux)
+        return [E.subst(e, aux) for e in self.model.drift_exprs()]
+
+    def check_adjoint(self):
+        """The hand-written adjoint covers: one coupled observable, a
+        post-summation linear in the sum, pre free of post_syn, additive
+        noise (diffs independent of the state), the dsl_cuda step order and
+        a float32 history."""
+        mstate, minput, _ = self.model.names
+        if self.C != 1:
+            raise ValueError('adjoint: one coupled observable only')
+        if self.order != 'cuda':
+            raise ValueError("adjoint: order='cuda' only")
+        if self.ring_dtype != 'f32':
+            raise ValueError('adjoint: float32 history only')
+        if self.cfun.pre_uses_post_syn():
+            raise ValueError('adjoint: pre_sum must not use post_syn')
+        post = E.subst(self.cfun.post_expr(0), {'mean': E.Bin(
+            '/', E.Var('sum'), E.Var('nnode'))})
+        if 'sum' in E.free_vars(E.diff(post, 'sum')):
+            raise ValueError('adjoint: post_sum must be linear in the sum')
+        if self.sigma_kind is None:
+            for g in self.model.diffs_exprs():
+                if E.free_vars(g) & set(mstate):
+                    raise ValueError('adjoint: multiplicative noise')
+
+    def emit_adjoint(self):
+        self.check_adjoint()
+        mstate, minput, _ = self.model.names
+        S = self.S
+        mn = self._model_names(True)
+        cn = self._cfun_names()
+        gc, gm = self.grad_params()
+        drift = self._drift_inlined()
+        lines = ['#define TVBX_NGC %d' % max(len(gc), 1),
+                 '#define TVBX_NGM %d' % max(len(gm), 1),
+                 '#define TVBX_HAS_NGC %d' % len(gc),
+                 '#define TVBX_HAS_NGM %d' % len(gm)]
+
+        def c_of(e, names):
+            return E.to_c(E.simplify(e), names)
+
+        # vector-Jacobian products of the drift
+        lines.append('__device__ __forceinline__ void tvbx_dfun_vjp('
+                     'const float *x, const float *c, const float *lam, '
+                     'float *jt, float *mu, float *gm, '
+                     'const tvbx_lane_params &lp, const tvbx_step_args &a, '
+                     'int i, int p) {')
+        for r, xr in enumerate(mstate):
+            terms = []
+            for s in range(S):
+                d = E.diff(drift[s], xr)
+                if not (isinstance(d, E.Num) and float(d.value) == 0):
+                    terms.append('lam[%d] * %s' % (s, c_of(d, mn)))
+            lines.append('  jt[%d] = %s;' % (r, ' + '.join(terms) or '0.0f'))
+        for k, ck in enumerate(minput[:self.C]):
+            terms = []
+            for s in range(S):
+                d = E.diff(drift[s], ck)
+                if not (isinstance(d, E.Num) and float(d.value) == 0):
+                    terms.append('lam[%d] * %s' % (s, c_of(d, mn)))
+            lines.append('  mu[%d] = %s;' % (k, ' + '.join(terms) or '0.0f'))
+        for q, key in enumerate(gm):
+            name = key.split(':', 1)[1]
+            terms = []
+            for s in range(S):
+                d = E.diff(drift[s], name)
+                if not (isinstance(d, E.Num) and float(d.value) == 0):
+                    terms.append('lam[%d] * %s' % (s, c_of(d, mn)))
+            lines.append('  gm[%d] = %s;' % (q, ' + '.join(terms) or '0.0f'))
+        if not gm:
+            lines.append('  gm[0] = 0.0f;')
+        lines.append('}')
+        # d h_k / d x
+        on = {k: v for k, v in mn.items() if not k.startswith('aux')}
+        for lhs, _ in self.model.auxex_exprs():
+            on.pop(lhs, None)
+        lines.append('__device__ __forceinline__ void tvbx_dobsrv(int k, '
+                     'const float *x, float *dh, const tvbx_lane_params &lp, '
+                     'const tvbx_step_args &a, int i, int p) {')
+        for k, h in enumerate(self.model.obsrv_exprs()):
+            lines.append('  if (k == %d) {' % k)
+            for r, xr in enumerate(mstate):
+                lines.append('    dh[%d] = %s;' % (r, c_of(E.diff(h, xr), on)))
+            lines.append('    return;')
+            lines.append('  }')
+        lines.append('}')
+        # pre' and the post derivatives
+        pre = self.cfun.pre_expr(0)
+        lines.append('__device__ __forceinline__ float tvbx_dpre(float y, '
+                     'const tvbx_lane_params &lp, const tvbx_step_args &a) {')
+        lines.append('  return %s;' % c_of(E.diff(pre, 'pre_syn'),
+                                           dict(cn, pre_syn='y')))
+        lines.append('}')
+        post = E.subst(self.cfun.post_expr(0), {'mean': E.Bin(
+            '/', E.Var('sum'), E.Var('nnode'))})
+        pn = dict(cn, sum='sum', nnode='(float)a.n_node')
+        lines.append('__device__ __forceinline__ float tvbx_dpost_dsum('
+                     'const tvbx_lane_params &lp, const tvbx_step_args &a) {')
+        lines.append('  const float sum = 0.0f; (void)sum;')
+        lines.append('  return %s;' % c_of(E.diff(post, 'sum'), pn))
+        lines.append('}')
+        lines.append('__device__ __forceinline__ void tvbx_dpost_dparam('
+                     'float sum, float *g, const tvbx_lane_params &lp, '
+                     'const tvbx_step_args &a) {')
+        for q, key in enumerate(gc):
+            name = key.split(':', 1)[1]
+            lines.append('  g[%d] = %s;' % (q, c_of(E.diff(post, name), pn)))
+        if not gc:
+            lines.append('  g[0] = 0.0f;')
+        lines.append('}')
+        return '\n'.join(lines) + '\n'
+
     def source(self):
         head = ('// generated by tvb_hpc.emit for %s / %s / %s\n'
                 % (type(self.model).__name__, type(self.cfun).__name__,
                    type(self.scheme).__name__))
-        return (head + _read('tvbx_args.h') + '\n' + _read('tvbx_philox.h')
-                + '\n' + self.emit_model() + '\n'
-                + _read('step_template.hip'))
+        src = (head + _read('tvbx_args.h') + '\n' + _read('tvbx_philox.h')
+               + '\n' + self.emit_model() + '\n'
+               + _read('step_template.hip'))
+        if self.adjoint:
+            src += '\n' + self.emit_adjoint() + '\n' + \
+                _read('adjoint_template.hip')
+        return src
 
     def key(self):
         h = hashlib.sha256(self.source().encode())

@@ -47,7 +47,7 @@ class SimDesc(ctypes.Structure):
         [('dt', ctypes.c_float)] + [(n, ctypes.c_int) for n in (
             'scheme', 'tavg_period', 'bold', 'bold_obs', 'fc', 'sample_slots',
             'waves', 'node_block', 'steps_per_launch', 'compat_outputs',
-            'order')]
+            'order', 'ipl', 'ring_type')] + [('ring_scale', ctypes.c_float)]
 
 
 _P = ctypes.c_void_p
@@ -80,6 +80,9 @@ _SIGS = {
     'tvbx_sim_init_history': [_P, _I],
     'tvbx_sim_set_noise': [_P, _I, ctypes.c_ulonglong],
     'tvbx_sim_run': [_P, _LL, _P, _LL],
+    'tvbx_sim_set_exec': [_P, _I, _I, _I],
+    'tvbx_sim_checkpoint_run': [_P, _LL, _I],
+    'tvbx_sim_adjoint': [_P, _P, _I, _LL, _LL, _I, _P, _I, _P, _I],
     'tvbx_sim_get_outputs': [_P, _P, _P, _P],
     'tvbx_sim_get_samples': [_P, _P, _P, _P],
     'tvbx_sim_fc': [_P, _I, _I, _P, _P],
@@ -260,7 +263,8 @@ class Simulation:
                  ring_pre=None, tavg_period=0, bold=False, bold_obs=0,
                  fc=False, sample_slots=64, waves=4, node_block=None,
                  steps_per_launch=1, compat_outputs=False, dt=None,
-                 order='cuda', context=None):
+                 order='cuda', ipl=1, ring_dtype='f32', ring_scale=1.0,
+                 adjoint=False, context=None):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.csr = csr
         N, P = csr.nnode, int(n_inst)
@@ -281,7 +285,10 @@ class Simulation:
             sigma_kind = 'const'
         self.spec = emit.KernelSpec(model, cfun, scheme, kinds=kinds,
                                     sigma_kind=sigma_kind, ring=ring,
-                                    ring_pre=ring_pre, order=order)
+                                    ring_pre=ring_pre, order=order, ipl=ipl,
+                                    ring_dtype=ring_dtype, adjoint=adjoint)
+        if ring_scale != 1.0 and not self.spec.ring_pre:
+            raise ValueError('ring_scale needs a ring that holds pre(y)')
         self.values = resolved
         self.ctx = context or Context.default()
         self.module = self.ctx.module(self.spec)
@@ -290,13 +297,15 @@ class Simulation:
             raise ValueError('horizon %d too short for max delay %d' %
                              (H, csr.max_delay))
         self.H = H
-        pitch = (P + 63) // 64 * 64
+        tile = 64 * int(ipl)
+        pitch = (P + tile - 1) // tile * tile
+        self.pitch = pitch
         if group_size is None:
             if csr.n_group != 1:
                 raise ValueError('several delay groups need group_size')
             group_size = pitch
-        if group_size % 64:
-            raise ValueError('group_size must be a multiple of 64')
+        if group_size % tile:
+            raise ValueError('group_size must be a multiple of 64 * ipl')
         if csr.n_group * group_size < pitch:
             raise ValueError('delay groups do not cover all instances')
         counts = self.spec.counts()
@@ -318,6 +327,9 @@ class Simulation:
         d.steps_per_launch = int(steps_per_launch)
         d.compat_outputs = int(bool(compat_outputs))
         d.order = 1 if order == 'reference' else 0
+        d.ipl = int(ipl)
+        d.ring_type = emit.RING_TYPES[ring_dtype]
+        d.ring_scale = float(ring_scale)
         self.desc = d
         self.h = ctypes.c_void_p()
         check(lib().tvbx_sim_create(self.ctx.h, self.module, ctypes.byref(d),
@@ -425,6 +437,12 @@ class Simulation:
         else:
             check(lib().tvbx_sim_set_noise(self.h, 1, int(seed)))
 
+    def set_exec(self, graph=True, window=100, profile=False):
+        """graph: replay hipGraph windows of `window` steps; profile: time
+        every launch (or window) with events for ``timing()``."""
+        check(lib().tvbx_sim_set_exec(self.h, int(bool(graph)), int(window),
+                                      int(bool(profile))))
+
     def run(self, n_steps, noise=None):
         """Advance n_steps.  noise: injected normals (n_steps, S, N, P) or
         (S, N, P) reused every step."""
@@ -437,6 +455,37 @@ class Simulation:
 
     def sync(self):
         self.ctx.sync()
+
+    # -- gradients ------------------------------------------------------------
+
+    def checkpoint_run(self, n_steps, window):
+        """Advance n_steps (a multiple of window) keeping checkpoints for
+        ``adjoint`` (requires ``adjoint=True``)."""
+        check(lib().tvbx_sim_checkpoint_run(self.h, int(n_steps),
+                                            int(window)))
+
+    def adjoint(self, dsamples, sample0=0):
+        """Gradients of a loss from dL/dsample ((n, N, P), numpy) of the
+        samples produced by the last ``checkpoint_run``.  Returns
+        {param key: gradient}: coupling parameters (P,), model parameters
+        of kind 'ni' (N, P) and 'inst' (P,)."""
+        gc, gm = self.spec.grad_params()
+        ds = np.ascontiguousarray(dsamples, np.float32)
+        n = ds.shape[0]
+        if ds.shape[1:] != (self.N, self.P):
+            raise ValueError('dsamples must be (n, N, P)')
+        g_c = np.zeros((max(len(gc), 1), self.P), np.float32)
+        g_m = np.zeros((max(len(gm), 1), self.N, self.P), np.float32)
+        check(lib().tvbx_sim_adjoint(self.h, _ptr(ds), 0, int(sample0), n,
+                                     len(gc), _ptr(g_c), len(gm), _ptr(g_m),
+                                     0))
+        out = {}
+        for k, key in enumerate(gc):
+            out[key] = g_c[k]
+        for k, key in enumerate(gm):
+            out[key] = g_m[k] if self.spec.kinds[key] == 'ni' else \
+                g_m[k].sum(axis=0)
+        return out
 
     def outputs(self):
         C, S, N, P = self.spec.C, self.spec.S, self.N, self.P
@@ -585,7 +634,7 @@ class NetworkStepExecutor:
         state[...] = _unbatch(np.transpose(xs, (1, 0, 2)), batched)
         obsrv[...] = _unbatch(np.transpose(r, (0, 2, 1, 3)), batched)
         if input is not None and np.size(input):
-            input[..., :inp.shape[0]] = _unbatch(np.transpose(inp, (1, 0, 2)),
+            input[:, :inp.shape[0]] = _unbatch(np.transpose(inp, (1, 0, 2)),
                                                  batched)
         if drift is not None and np.size(drift):
             drift[...] = _unbatch(np.transpose(dr, (1, 0, 2)), batched)

@@ -89,6 +89,21 @@ def lengths_to_delays(lengths, speed, dt, policy='trunc'):
     return d.astype(np.uintc)
 
 
+def bin_delays(delays, n_bins):
+    """Quantise integer delays to at most n_bins distinct values (equal-width
+    bins over [min, max], each delay mapped to its bin centre rounded to a
+    step).  Returns (quantised delays, bin centres)."""
+    d = np.asarray(delays, np.int64)
+    lo, hi = int(d.min()), int(d.max())
+    if hi - lo + 1 <= n_bins:
+        return d.astype(np.int32), np.unique(d).astype(np.int32)
+    edges = np.linspace(lo, hi + 1, n_bins + 1)
+    idx = np.clip(np.searchsorted(edges, d, side='right') - 1, 0, n_bins - 1)
+    centres = np.clip(np.round((edges[:-1] + edges[1:] - 1) / 2), lo, hi)
+    centres = centres.astype(np.int32)
+    return centres[idx].astype(np.int32), centres
+
+
 def horizon_for(max_delay: int) -> int:
     """Ring length for a maximum delay: slot n+1 is written while slots
     n - d (0 <= d <= max_delay) are read, so the ring needs max_delay + 2
@@ -206,6 +221,27 @@ class CSR:
         speeds = np.atleast_1d(speeds)
         d = np.stack([conn.delays(s, dt, policy) for s in speeds])
         return cls(conn.nnode, conn.row, conn.col, conn.wnz, d)
+
+    def binned(self, n_bins=32):
+        """Delays quantised to ``n_bins`` values: equal-width bins over
+        [min, max] of all groups, each delay replaced by its bin's centre
+        (rounded to a step).  The error per edge is at most half a bin.
+        Returns (CSR, bin centres)."""
+        d, centres = bin_delays(self.delays, n_bins)
+        return CSR(self.nnode, self.row, self.col, self.weight, d), centres
+
+    def sorted_by_delay(self):
+        """The same connectome with each row's edges ordered by (delay of
+        group 0, source): consecutive gathers of a row then read nearby
+        ring slots.  (Changes the summation order, not the sum's terms.)"""
+        perm = np.arange(self.nnz)
+        for i in range(self.nnode):
+            a, b = self.row[i], self.row[i + 1]
+            if b - a > 1:
+                key = np.lexsort((self.col[a:b], self.delays[0, a:b]))
+                perm[a:b] = a + key
+        return CSR(self.nnode, self.row, self.col[perm], self.weight[perm],
+                   self.delays[:, perm])
 
     def dense_weights(self):
         W = np.zeros((self.nnode, self.nnode), np.float64)
