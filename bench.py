@@ -97,7 +97,23 @@ def main():
     ap.add_argument('--launch', default=None,
                     help='ipl,waves,node_block,steps_per_launch '
                          '(default: tuned)')
+    ap.add_argument('--no-graph', action='store_true',
+                    help='time direct launches instead of graph windows')
+    ap.add_argument('--copy-calibration', type=int, default=0,
+                    metavar='MIB',
+                    help='only run 10 elementwise copies of MIB MiB (the '
+                         'known-size kernel that scales PMC byte counters)')
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, device='cuda:0')
+        y = torch.empty_like(x)
+        for _ in range(10):
+            torch.mul(x, 1.0, out=y)   # reads and writes 4 n bytes
+        torch.cuda.synchronize()
+        log('copied %d MiB x 10' % args.copy_calibration)
+        return
     if args.gpus != 1:
         print('bench.py measures one GPU: the integrator shards sweep '
               'instances across GPUs without communication, and this build '
@@ -126,7 +142,8 @@ def main():
     sigma = getattr(cfg.scheme, 'sigma', None)
     if sigma is not None:
         sim.set_noise(seed=cfg.seed + 1)
-    sim.set_exec(graph=True, window=args.graph_window, profile=False)
+    sim.set_exec(graph=not args.no_graph, window=args.graph_window,
+                 profile=False)
     sim.run(W)
     sim.sync()
     t0 = time.perf_counter()
@@ -150,7 +167,7 @@ def main():
     steps_per_launch = K_prof / max(launches, 1)
     achieved = bytes_inst_step * P * steps_per_launch / (
         ms_kern / launches * 1e-3) / 1e9
-    traffic = traffic_from_profile('tvbx_step', cfg.name)
+    traffic = traffic_from_profile('tvbx_step', args.workload)
     out = {
         'metric': METRIC,
         'value': value,

@@ -277,14 +277,18 @@ def test_monitor_bold_samples_match_oracle():
     s_dev, first = sim.samples()
     assert first == 0 and s_dev.shape == (100, 16, 64)
     s_ref = out['samples']
-    err = np.array([scaled_err(s_dev[..., p], s_ref[..., p])
-                    for p in range(64)])
-    print('worst sample error per instance:', np.sort(err)[-4:],
-          'at', np.argsort(err)[-4:])
-    if err.max() >= 1e-4 and os.path.isdir('gpurun_out'):
+    # BOLD = V0 (k1 (1 - q) + k2 (1 - q/v) + k3 (1 - v)) with q, v ~ 1 and
+    # V0 k1 ~ 11: one float32 ulp of q (1.2e-7, from powf on either side)
+    # moves the signal by 1.3e-6, whatever the signal's own size.
+    err = np.abs(s_dev - s_ref)
+    print('max |dBOLD| %.2e of max |BOLD| %.2e' % (err.max(),
+                                                   np.abs(s_ref).max()))
+    if err.max() >= 4e-6 and os.path.isdir('gpurun_out'):
         np.savez('gpurun_out/monitor_samples.npz', dev=s_dev, ref=s_ref,
                  x_dev=sim.get_state(), x_ref=ref.get_state())
-    assert err.max() < 1e-4
+    np.testing.assert_array_equal(sim.get_state(), ref.get_state())
+    assert err.max() < 4e-6
+    assert err.max() < 2e-3 * np.abs(s_ref).max()
 
 
 def test_device_fc_matches_fp64_fc_of_samples():
