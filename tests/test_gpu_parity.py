@@ -194,6 +194,88 @@ def test_graph_windows_match_eager_launches(launch):
     np.testing.assert_array_equal(simG.get_state(), simE.get_state())
 
 
+def _block_len(dt, cap=8):
+    "longest temporal block the small test connectome allows (<= cap)"
+    return min(int(small_csr(N=24, dt=dt).delays.min()) + 1, cap)
+
+
+@pytest.mark.parametrize('model,cfun,dt,cparams', CASES)
+@pytest.mark.parametrize('ipl', [1, 4])
+def test_temporal_blocking_matches_per_step_launches(model, cfun, dt, cparams,
+                                                     ipl):
+    """Launches of d_min + 1 steps with one node per wave and no
+    synchronisation reproduce one-step launches bit for bit (same arithmetic,
+    state kept in registers instead of HBM), and the oracle within TOL."""
+    L = _block_len(dt)
+    assert L > 1
+    P = 300
+    simA, ref, noise = _pair(model, cfun, dt, cparams, P=P, steps=600,
+                             launch=dict(ipl=ipl, waves=4, node_block=4))
+    simB, _, _ = _pair(model, cfun, dt, cparams, P=P, steps=600,
+                       launch=dict(ipl=ipl, waves=4, node_block=4,
+                                   steps_per_launch=L))
+    assert simB.blocked and simB.H == simA.H + L - 1
+    simA.run(600, noise=noise)
+    simB.run(600, noise=noise)
+    np.testing.assert_array_equal(simB.get_state(), simA.get_state())
+    ref.run(600, noise=noise)
+    assert state_err(model, simB.get_state(), ref.get_state()) < TOL
+    # Philox noise through graph windows (window not a multiple of L: the
+    # window is rounded to one)
+    for s in (simA, simB):
+        s.set_noise(seed=5)
+    simB.set_exec(graph=True, window=50)
+    simA.set_exec(graph=False)
+    simA.run(3 * 56)
+    simB.run(3 * 56)
+    np.testing.assert_array_equal(simB.get_state(), simA.get_state())
+    assert not simB.flags().any()
+
+
+def test_temporal_blocking_monitor_samples():
+    """tavg + BOLD samples and FC of blocked launches equal those of
+    one-step launches."""
+    dt = 0.1
+    L = _block_len(dt)
+    sims = []
+    for launch in (dict(waves=4, node_block=4),
+                   dict(waves=4, node_block=4, steps_per_launch=L)):
+        sim, _ = _monitor_pair(fc=False, slots=100, launch=launch)
+        sim.run(2000)
+        sims.append(sim)
+    a, fa = sims[0].samples()
+    b, fb = sims[1].samples()
+    assert fa == fb == 0
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(sims[0].get_state(), sims[1].get_state())
+
+
+def test_temporal_blocking_validation():
+    dt = 0.1
+    csr = small_csr(N=24, dt=dt)
+    dmin = int(csr.delays.min())
+    m = MODELS['Linear']()
+    c = CFUNS['Linear'](m)
+    sc = SC.EulerStep(dt)
+    with pytest.raises(ValueError):    # a block longer than d_min + 1
+        Simulation(m, c, sc, csr, 64, waves=4, node_block=4,
+                   steps_per_launch=dmin + 2)
+    with pytest.raises(ValueError):    # not one node per wave
+        Simulation(m, c, sc, csr, 64, waves=4, node_block=8,
+                   steps_per_launch=2)
+    with pytest.raises(ValueError):    # the loopy ordering
+        Simulation(m, c, sc, csr, 64, waves=4, node_block=4,
+                   steps_per_launch=2, order='reference')
+    zd = small_csr(N=24, dt=dt, zero_delays=3)
+    with pytest.raises(ValueError):    # zero delays
+        Simulation(m, c, sc, zd, 64, waves=4, node_block=4,
+                   steps_per_launch=2)
+    # the C ABI checks the delays against the horizon on its own
+    sim = Simulation(m, c, sc, csr, 64, waves=4, node_block=4,
+                     steps_per_launch=dmin + 1)
+    assert sim.H == csr.max_delay + dmin + 2
+
+
 def test_rng_normals_match_oracle(ctx):
     z = ctx.rng_normals(1 << 16, seed=99)
     zr = O.normals_f32(1 << 16, seed=99)
@@ -242,7 +324,7 @@ def test_delay_groups_match_oracle():
     assert scaled_err(sim.get_state(), ref.get_state()) < TOL
 
 
-def _monitor_pair(fc, slots):
+def _monitor_pair(fc, slots, launch=None):
     dt = 0.1
     csr = small_csr(N=16, dt=dt)
     P = 64
@@ -251,7 +333,8 @@ def _monitor_pair(fc, slots):
     sc = SC.EulerMaryuyamaStep(dt, sigma=0.005)
     G = np.linspace(0, 1, P).astype(np.float32)
     sim = Simulation(m, c, sc, csr, P, params={'c:a': G}, tavg_period=20,
-                     bold=True, fc=fc, sample_slots=slots, waves=4)
+                     bold=True, fc=fc, sample_slots=slots,
+                     **(launch or dict(waves=4)))
     x0 = m.initial_state(16, P, np.random.default_rng(5))
     sim.set_state(x0)
     sim.init_history('zeros')

@@ -9,6 +9,7 @@
 //   TVBX_ORDER_REF   1: the loopy kernel's read-before-write order
 //   TVBX_IPL         sweep instances per lane (1, 2 or 4)
 //   TVBX_RING_T      ring element: 0 float32, 1 bfloat16, 2 float16
+//   TVBX_UNROLL      delayed gathers issued together per CSR row group
 //   struct tvbx_lane_params + tvbx_load_lane()   per-instance parameters
 //   tvbx_pre(), tvbx_post(), tvbx_dfun(), tvbx_obsrv(), tvbx_wrap(),
 //   tvbx_ring_value(), tvbx_bold_step()
@@ -34,9 +35,6 @@
 // ring may be bfloat16 or float16 (round to nearest even), in which case it
 // stores ring_scale * pre(y) and the sum is multiplied back by ring_unscale.
 
-#ifndef TVBX_UNROLL
-#define TVBX_UNROLL 8
-#endif
 
 #if TVBX_RING_T == 0
 typedef float tvbx_rt;
@@ -134,6 +132,57 @@ __device__ __forceinline__ void tvbx_str(tvbx_rt *__restrict__ p,
 #endif
 }
 
+// TVBX_IPL consecutive ring elements as loaded (one vector register group),
+// unpacked to float32 only where they are consumed, so that TVBX_UNROLL
+// gathers in flight cost 1 (16-bit) or 2 (float32) registers per element
+#if TVBX_RING_T == 0
+#if TVBX_IPL == 4
+typedef float4 tvbx_raw;
+#elif TVBX_IPL == 2
+typedef float2 tvbx_raw;
+#else
+typedef float tvbx_raw;
+#endif
+#else
+#if TVBX_IPL == 4
+typedef uint2 tvbx_raw;
+#elif TVBX_IPL == 2
+typedef unsigned tvbx_raw;
+#else
+typedef unsigned short tvbx_raw;
+#endif
+#endif
+
+__device__ __forceinline__ tvbx_raw tvbx_ldraw(const tvbx_rt *__restrict__ p) {
+  return *reinterpret_cast<const tvbx_raw *>(p);
+}
+
+__device__ __forceinline__ void tvbx_unpack(const tvbx_raw &v,
+                                            float (&o)[TVBX_IPL]) {
+#if TVBX_RING_T == 0
+#if TVBX_IPL == 4
+  o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+#elif TVBX_IPL == 2
+  o[0] = v.x; o[1] = v.y;
+#else
+  o[0] = v;
+#endif
+#else
+#if TVBX_IPL == 4
+  const unsigned w[2] = {v.x, v.y};
+#elif TVBX_IPL == 2
+  const unsigned w[1] = {v};
+#else
+  const unsigned w[1] = {(unsigned)v};
+#endif
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q) {
+    const unsigned short h = (unsigned short)(w[q >> 1] >> (16 * (q & 1)));
+    o[q] = tvbx_r2f(*reinterpret_cast<const tvbx_rt *>(&h));
+  }
+#endif
+}
+
 // normals for state variables [s0, s0 + 4) of (instance p, step n, node i)
 __device__ __forceinline__ void tvbx_normals4(const tvbx_step_args &a,
                                               long long n, int i, int p,
@@ -158,23 +207,216 @@ __device__ __forceinline__ size_t tvbx_ring_off(int slot, int k, int j,
   return (size_t)((slot * TVBX_R + k) * a.n_node + j) * (size_t)a.pitch;
 }
 
-// y_j(n - d) of coupled observable kk for this lane's instances.  In the
-// reference (loopy) ordering a zero delay reads what slot n held before step
-// n rewrote it (st_cur).  d is wave-uniform: the branch is a scalar one.
-__device__ __forceinline__ void tvbx_delayed(const tvbx_rt *__restrict__ ring,
-                                             const tvbx_rt *__restrict__ st_cur,
-                                             int t, int d, int kk, int j,
-                                             int pb, const tvbx_step_args &a,
-                                             float (&o)[TVBX_IPL]) {
+// y_j(n - d) of coupled observable kk for this lane's instances, as loaded.
+// In the reference (loopy) ordering a zero delay reads what slot n held
+// before step n rewrote it (st_cur).  d is wave-uniform: the branch is a
+// scalar one.
+__device__ __forceinline__ tvbx_raw tvbx_delayed(
+    const tvbx_rt *__restrict__ ring, const tvbx_rt *__restrict__ st_cur,
+    int t, int d, int kk, int j, int pb, const tvbx_step_args &a) {
   int slot = t - d;
   slot += (slot < 0) ? a.horizon : 0;
 #if TVBX_ORDER_REF
-  if (d == 0) {
-    tvbx_ldr(st_cur + (size_t)(kk * a.n_node + j) * a.pitch + pb, o);
-    return;
-  }
+  if (d == 0)
+    return tvbx_ldraw(st_cur + (size_t)(kk * a.n_node + j) * a.pitch + pb);
 #endif
-  tvbx_ldr(ring + tvbx_ring_off(slot, kk, j, a) + pb, o);
+  return tvbx_ldraw(ring + tvbx_ring_off(slot, kk, j, a) + pb);
+}
+
+// Coupling sums c[q][kk] of node i at step n (ring slot t): the CSR row
+// reduction of network.py:115-136, with rec_sum recording for the adjoint.
+__device__ __forceinline__ void tvbx_coupling(
+    const tvbx_step_args &a, const unsigned *__restrict__ cd,
+    const tvbx_rt *__restrict__ ring, const tvbx_rt *__restrict__ st_cur,
+    int i, int t, int pb, long long n, const tvbx_lane_params (&lp)[TVBX_IPL],
+    float (&c)[TVBX_IPL][TVBX_C]) {
+  const size_t NP = (size_t)a.n_node * a.pitch;
+  const size_t ip = (size_t)i * a.pitch + pb;
+  const float *__restrict__ wt = a.weight;
+  const int e0 = a.row[i], e1 = a.row[i + 1];
+#pragma unroll
+  for (int kk = 0; kk < TVBX_C; ++kk) {
+    float post[TVBX_IPL], acc[TVBX_IPL];
+#pragma unroll
+    for (int q = 0; q < TVBX_IPL; ++q) {
+      post[q] = 0.0f;
+      acc[q] = 0.0f;
+    }
+    if (TVBX_USES_POST) {
+#if TVBX_ORDER_REF
+      tvbx_ldr(st_cur + kk * NP + ip, post);
+#else
+      tvbx_ldr(ring + tvbx_ring_off(t, kk, i, a) + pb, post);
+#endif
+    }
+    int e = e0;
+    for (; e + TVBX_UNROLL <= e1; e += TVBX_UNROLL) {
+      tvbx_raw v[TVBX_UNROLL];
+#pragma unroll
+      for (int u = 0; u < TVBX_UNROLL; ++u) {
+        const unsigned w = cd[e + u];
+        v[u] = tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,
+                            (int)(w & 0xffffu), pb, a);
+      }
+#pragma unroll
+      for (int u = 0; u < TVBX_UNROLL; ++u) {
+        const float wu = wt[e + u];
+        float f[TVBX_IPL];
+        tvbx_unpack(v[u], f);
+#pragma unroll
+        for (int q = 0; q < TVBX_IPL; ++q)
+          acc[q] = fmaf(wu, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);
+      }
+    }
+    for (; e < e1; ++e) {
+      const unsigned w = cd[e];
+      float f[TVBX_IPL];
+      tvbx_unpack(tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,
+                               (int)(w & 0xffffu), pb, a),
+                  f);
+      const float we = wt[e];
+#pragma unroll
+      for (int q = 0; q < TVBX_IPL; ++q)
+        acc[q] = fmaf(we, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < TVBX_IPL; ++q) {
+      acc[q] = acc[q] * a.ring_unscale;
+      c[q][kk] = tvbx_post(kk, acc[q], lp[q], a);
+    }
+    if (a.rec_sum != nullptr)
+      tvbx_stf(a.rec_sum + ((size_t)(n - a.rec_step0) * TVBX_C + kk) * NP + ip,
+               acc);
+  }
+}
+
+// dfun (model.py:105-140), Euler / Euler-Maruyama update (scheme.py:82-97),
+// wrap, non-finite flags and the ring values of x(n + 1).  x is updated in
+// place; k is the step's index within the launch.
+__device__ __forceinline__ void tvbx_advance(
+    const tvbx_step_args &a, int i, int pb, long long n, int k,
+    const tvbx_lane_params (&lp)[TVBX_IPL], float (&x)[TVBX_IPL][TVBX_S],
+    const float (&c)[TVBX_IPL][TVBX_C], float (&yo)[TVBX_R][TVBX_IPL],
+    unsigned &bad) {
+  const size_t NP = (size_t)a.n_node * a.pitch;
+  const size_t ip = (size_t)i * a.pitch + pb;
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q) {
+    const int p = pb + q;
+    float f[TVBX_S], g[TVBX_S], y[TVBX_NOBS];
+    tvbx_dfun(x[q], c[q], f, g, lp[q], a, i, p);
+#if TVBX_EM
+    float xi[TVBX_S];
+    if (a.noise_mode == 1) {
+#pragma unroll
+      for (int s0 = 0; s0 < TVBX_S; s0 += 4) {
+        float z[4];
+        tvbx_normals4(a, n, i, p, s0, z);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (s0 + r < TVBX_S) xi[s0 + r] = z[r];
+      }
+    } else if (a.noise_mode == 2) {
+      const float *nz = a.noise + (size_t)k * a.noise_tstride;
+#pragma unroll
+      for (int s = 0; s < TVBX_S; ++s) xi[s] = nz[s * NP + ip + q];
+    } else {
+#pragma unroll
+      for (int s = 0; s < TVBX_S; ++s) xi[s] = 0.0f;
+    }
+#pragma unroll
+    for (int s = 0; s < TVBX_S; ++s)
+      x[q][s] = x[q][s] + a.dt * f[s] + a.sqrt_dt * xi[s] * g[s];
+#else
+#pragma unroll
+    for (int s = 0; s < TVBX_S; ++s) x[q][s] = x[q][s] + a.dt * f[s];
+#endif
+    tvbx_wrap(x[q]);
+#pragma unroll
+    for (int s = 0; s < TVBX_S; ++s)
+      bad |= (unsigned)(!isfinite(x[q][s]) && p < a.n_inst) << q;
+    // observables of x(n + 1) for the history
+    tvbx_obsrv(x[q], y, lp[q], a, i, p);
+#pragma unroll
+    for (int r = 0; r < TVBX_R; ++r) {
+      float v = tvbx_ring_value(r, y[r], lp[q], a);
+      if (TVBX_RING_PRE && r < TVBX_C) v = v * a.ring_scale;
+      yo[r][q] = v;
+    }
+    // compatibility outputs of the last step of the launch
+    if (a.out_input != nullptr && k == a.n_steps - 1) {
+#pragma unroll
+      for (int kk = 0; kk < TVBX_C; ++kk)
+        a.out_input[kk * NP + ip + q] = c[q][kk];
+#pragma unroll
+      for (int s = 0; s < TVBX_S; ++s) {
+        a.out_drift[s * NP + ip + q] = f[s];
+        a.out_diffs[s * NP + ip + q] = g[s];
+      }
+    }
+  }
+}
+
+// monitor observable of the state at the start of the step
+__device__ __forceinline__ void tvbx_monitor_obs(
+    const tvbx_step_args &a, int i, int pb,
+    const tvbx_lane_params (&lp)[TVBX_IPL], const float (&x)[TVBX_IPL][TVBX_S],
+    float (&y_mon)[TVBX_IPL]) {
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q) {
+    float y0[TVBX_NOBS];
+    tvbx_obsrv(x[q], y0, lp[q], a, i, pb + q);
+    y_mon[q] = tvbx_pick(y0, a.bold_obs);
+  }
+}
+
+// time average over tavg_period steps; at the period's last step the
+// average (optionally through Balloon-Windkessel) goes to sample slot
+// m mod sample_slots and the accumulator restarts
+__device__ __forceinline__ void tvbx_monitor(const tvbx_step_args &a,
+                                             size_t ip, long long n,
+                                             const float (&y_mon)[TVBX_IPL],
+                                             float (&acc_t)[TVBX_IPL]) {
+  const size_t NP = (size_t)a.n_node * a.pitch;
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q) acc_t[q] = acc_t[q] + y_mon[q];
+  if (((n + 1) % a.tavg_period) == 0) {
+    float out[TVBX_IPL];
+#pragma unroll
+    for (int q = 0; q < TVBX_IPL; ++q) {
+      out[q] = acc_t[q] / (float)a.tavg_period;
+      if (a.bold) out[q] = tvbx_bold_step(a, ip + q, out[q]);
+      acc_t[q] = 0.0f;
+    }
+    const long long m = (n + 1) / a.tavg_period - 1;
+    tvbx_stf(a.samples + (size_t)(m % a.sample_slots) * NP + ip, out);
+  }
+}
+
+// x(n) for the adjoint's replay record
+__device__ __forceinline__ void tvbx_record_state(
+    const tvbx_step_args &a, size_t ip, long long n,
+    const float (&x)[TVBX_IPL][TVBX_S]) {
+  const size_t NP = (size_t)a.n_node * a.pitch;
+#pragma unroll
+  for (int s = 0; s < TVBX_S; ++s) {
+    float v[TVBX_IPL];
+#pragma unroll
+    for (int q = 0; q < TVBX_IPL; ++q) v[q] = x[q][s];
+    tvbx_stf(a.rec_x + ((size_t)(n - a.rec_step0) * TVBX_S + s) * NP + ip, v);
+  }
+}
+
+// step and ring slot of the launch's first step (graph replays add the
+// device clock)
+__device__ __forceinline__ void tvbx_start(const tvbx_step_args &a,
+                                           long long &n0, int &t) {
+  n0 = a.step0;
+  t = a.slot0;
+  if (a.clock != nullptr) {
+    n0 += a.clock[0];
+    t = (int)((a.clock[1] + a.slot0) % a.horizon);
+  }
 }
 
 extern "C" __global__ void __launch_bounds__(1024)
@@ -190,8 +432,6 @@ tvbx_step(const tvbx_step_args a) {
   const int group = __builtin_amdgcn_readfirstlane(
       (blockIdx.x * 64 * TVBX_IPL) / a.group_size);
   const unsigned *__restrict__ cd = a.coldelay + (size_t)group * a.nnz;
-  const float *__restrict__ wt = a.weight;
-  const int *__restrict__ row = a.row;
   const tvbx_rt *__restrict__ ring = (const tvbx_rt *)a.ring;
   tvbx_rt *ring_w = (tvbx_rt *)a.ring;
   const int i_begin = blockIdx.y * a.node_block;
@@ -201,17 +441,13 @@ tvbx_step(const tvbx_step_args a) {
 #pragma unroll
   for (int q = 0; q < TVBX_IPL; ++q) tvbx_load_lane(lp[q], a, pb + q);
   unsigned bad = 0u;
-  long long n0 = a.step0;
-  int t = a.slot0;
-  if (a.clock != nullptr) {  // graph replay: offsets from the device clock
-    n0 += a.clock[0];
-    t = (int)((a.clock[1] + a.slot0) % H);
-  }
+  long long n0;
+  int t;
+  tvbx_start(a, n0, t);
 
   for (int k = 0; k < a.n_steps; ++k) {
     const long long n = n0 + k;
     const int t1 = (t + 1 == H) ? 0 : t + 1;
-    const bool sample = a.tavg_period > 0 && ((n + 1) % a.tavg_period) == 0;
 #if TVBX_ORDER_REF
     const size_t RNP = (size_t)TVBX_R * NP;
     const tvbx_rt *st_cur = (const tvbx_rt *)a.stale + (size_t)(n & 1) * RNP;
@@ -229,143 +465,26 @@ tvbx_step(const tvbx_step_args a) {
         tvbx_ldf(a.state + s * NP + ip, v);
 #pragma unroll
         for (int q = 0; q < TVBX_IPL; ++q) x[q][s] = v[q];
-        if (a.rec_x != nullptr)
-          tvbx_stf(a.rec_x + ((size_t)(n - a.rec_step0) * TVBX_S + s) * NP + ip,
-                   v);
       }
-      // monitor input: observable of the state at the start of the step
+      if (a.rec_x != nullptr) tvbx_record_state(a, ip, n, x);
       float y_mon[TVBX_IPL];
-      if (a.tavg_period > 0) {
+      if (a.tavg_period > 0) tvbx_monitor_obs(a, i, pb, lp, x, y_mon);
+
+      tvbx_coupling(a, cd, ring, st_cur, i, t, pb, n, lp, c);
+
+      float yo[TVBX_R][TVBX_IPL];
+      tvbx_advance(a, i, pb, n, k, lp, x, c, yo, bad);
 #pragma unroll
-        for (int q = 0; q < TVBX_IPL; ++q) {
-          float y0[TVBX_NOBS];
-          tvbx_obsrv(x[q], y0, lp[q], a, i, pb + q);
-          y_mon[q] = tvbx_pick(y0, a.bold_obs);
-        }
+      for (int s = 0; s < TVBX_S; ++s) {
+        float v[TVBX_IPL];
+#pragma unroll
+        for (int q = 0; q < TVBX_IPL; ++q) v[q] = x[q][s];
+        tvbx_stf(a.state + s * NP + ip, v);
       }
 
-      // ---- delayed coupling: network.py:115-136
-      const int e0 = row[i], e1 = row[i + 1];
-#pragma unroll
-      for (int kk = 0; kk < TVBX_C; ++kk) {
-        float post[TVBX_IPL], acc[TVBX_IPL];
-#pragma unroll
-        for (int q = 0; q < TVBX_IPL; ++q) {
-          post[q] = 0.0f;
-          acc[q] = 0.0f;
-        }
-        if (TVBX_USES_POST) {
-#if TVBX_ORDER_REF
-          tvbx_ldr(st_cur + kk * NP + ip, post);
-#else
-          tvbx_ldr(ring + tvbx_ring_off(t, kk, i, a) + pb, post);
-#endif
-        }
-        int e = e0;
-        for (; e + TVBX_UNROLL <= e1; e += TVBX_UNROLL) {
-          float v[TVBX_UNROLL][TVBX_IPL];
-#pragma unroll
-          for (int u = 0; u < TVBX_UNROLL; ++u) {
-            const unsigned w = cd[e + u];
-            tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,
-                         (int)(w & 0xffffu), pb, a, v[u]);
-          }
-#pragma unroll
-          for (int u = 0; u < TVBX_UNROLL; ++u) {
-            const float wu = wt[e + u];
-#pragma unroll
-            for (int q = 0; q < TVBX_IPL; ++q)
-              acc[q] = fmaf(wu, tvbx_pre(kk, v[u][q], post[q], lp[q], a),
-                            acc[q]);
-          }
-        }
-        for (; e < e1; ++e) {
-          const unsigned w = cd[e];
-          float vv[TVBX_IPL];
-          tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,
-                       (int)(w & 0xffffu), pb, a, vv);
-          const float we = wt[e];
-#pragma unroll
-          for (int q = 0; q < TVBX_IPL; ++q)
-            acc[q] = fmaf(we, tvbx_pre(kk, vv[q], post[q], lp[q], a), acc[q]);
-        }
-#pragma unroll
-        for (int q = 0; q < TVBX_IPL; ++q) {
-          acc[q] = acc[q] * a.ring_unscale;
-          c[q][kk] = tvbx_post(kk, acc[q], lp[q], a);
-        }
-        if (a.rec_sum != nullptr)
-          tvbx_stf(a.rec_sum + ((size_t)(n - a.rec_step0) * TVBX_C + kk) * NP +
-                       ip,
-                   acc);
-      }
-
-      float xo[TVBX_S][TVBX_IPL], yo[TVBX_R][TVBX_IPL];
-#pragma unroll
-      for (int q = 0; q < TVBX_IPL; ++q) {
-        const int p = pb + q;
-        float f[TVBX_S], g[TVBX_S], y[TVBX_NOBS];
-        // ---- dfun: model.py:105-140
-        tvbx_dfun(x[q], c[q], f, g, lp[q], a, i, p);
-        // ---- scheme: scheme.py:82-97
-#if TVBX_EM
-        float xi[TVBX_S];
-        if (a.noise_mode == 1) {
-#pragma unroll
-          for (int s0 = 0; s0 < TVBX_S; s0 += 4) {
-            float z[4];
-            tvbx_normals4(a, n, i, p, s0, z);
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-              if (s0 + r < TVBX_S) xi[s0 + r] = z[r];
-          }
-        } else if (a.noise_mode == 2) {
-          const float *nz = a.noise + (size_t)k * a.noise_tstride;
-#pragma unroll
-          for (int s = 0; s < TVBX_S; ++s) xi[s] = nz[s * NP + ip + q];
-        } else {
-#pragma unroll
-          for (int s = 0; s < TVBX_S; ++s) xi[s] = 0.0f;
-        }
-#pragma unroll
-        for (int s = 0; s < TVBX_S; ++s)
-          x[q][s] = x[q][s] + a.dt * f[s] + a.sqrt_dt * xi[s] * g[s];
-#else
-#pragma unroll
-        for (int s = 0; s < TVBX_S; ++s) x[q][s] = x[q][s] + a.dt * f[s];
-#endif
-        tvbx_wrap(x[q]);
-#pragma unroll
-        for (int s = 0; s < TVBX_S; ++s) {
-          xo[s][q] = x[q][s];
-          bad |= (unsigned)(!isfinite(x[q][s]) && p < a.n_inst) << q;
-        }
-        // ---- observables of x(n + 1) for the history
-        tvbx_obsrv(x[q], y, lp[q], a, i, p);
-#pragma unroll
-        for (int r = 0; r < TVBX_R; ++r) {
-          float v = tvbx_ring_value(r, y[r], lp[q], a);
-          if (TVBX_RING_PRE && r < TVBX_C) v = v * a.ring_scale;
-          yo[r][q] = v;
-        }
-        // ---- compatibility outputs of the last step of the launch
-        if (a.out_input != nullptr && k == a.n_steps - 1) {
-#pragma unroll
-          for (int kk = 0; kk < TVBX_C; ++kk)
-            a.out_input[kk * NP + ip + q] = c[q][kk];
-#pragma unroll
-          for (int s = 0; s < TVBX_S; ++s) {
-            a.out_drift[s * NP + ip + q] = f[s];
-            a.out_diffs[s * NP + ip + q] = g[s];
-          }
-        }
-      }
-#pragma unroll
-      for (int s = 0; s < TVBX_S; ++s) tvbx_stf(a.state + s * NP + ip, xo[s]);
-
-      // ---- history write: y(n + 1) into slot t + 1 (in the reference
-      // ordering the slot's previous content moves to the stale block that
-      // step n + 1 reads for post_syn and zero delays)
+      // history write: y(n + 1) into slot t + 1 (in the reference ordering
+      // the slot's previous content moves to the stale block that step
+      // n + 1 reads for post_syn and zero delays)
 #pragma unroll
       for (int r = 0; r < TVBX_R; ++r) {
         tvbx_rt *dst = ring_w + tvbx_ring_off(t1, r, i, a) + pb;
@@ -379,23 +498,10 @@ tvbx_step(const tvbx_step_args a) {
         tvbx_str(dst, yo[r]);
       }
 
-      // ---- monitor: time average over tavg_period steps, optional BOLD
       if (a.tavg_period > 0) {
         float acc_t[TVBX_IPL];
         tvbx_ldf(a.tavg + ip, acc_t);
-#pragma unroll
-        for (int q = 0; q < TVBX_IPL; ++q) acc_t[q] = acc_t[q] + y_mon[q];
-        if (sample) {
-          float out[TVBX_IPL];
-#pragma unroll
-          for (int q = 0; q < TVBX_IPL; ++q) {
-            out[q] = acc_t[q] / (float)a.tavg_period;
-            if (a.bold) out[q] = tvbx_bold_step(a, ip + q, out[q]);
-            acc_t[q] = 0.0f;
-          }
-          const long long m = (n + 1) / a.tavg_period - 1;
-          tvbx_stf(a.samples + (size_t)(m % a.sample_slots) * NP + ip, out);
-        }
+        tvbx_monitor(a, ip, n, y_mon, acc_t);
         tvbx_stf(a.tavg + ip, acc_t);
       }
     }
@@ -406,6 +512,83 @@ tvbx_step(const tvbx_step_args a) {
   for (int q = 0; q < TVBX_IPL; ++q)
     if ((bad >> q) & 1u) a.bad[pb + q] = 1u;
 }
+
+#if !TVBX_ORDER_REF
+// Temporal blocking.  Every delay is at least d_min >= 1, so the n_steps
+// <= d_min + 1 steps of one launch read only ring slots written before the
+// launch (y(n - d) with n - d <= n0), apart from the node's own post-
+// synaptic value y_i(n), which this wave wrote at the previous step.  The
+// host also keeps horizon >= max delay + n_steps + 1 so the slots written in
+// the launch are never among those read.  Workgroups therefore need no
+// synchronisation inside a launch: each wave owns one node (node_block ==
+// waves) and keeps its state and tavg accumulator in registers for all the
+// launch's steps.  At most 8 waves per workgroup (the state of 4 instances
+// per lane needs more than 128 registers).
+extern "C" __global__ void __launch_bounds__(512)
+tvbx_step_blocked(const tvbx_step_args a) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int pb = (blockIdx.x * 64 + lane) * TVBX_IPL;
+  const int N = a.n_node;
+  const int H = a.horizon;
+  const size_t NP = (size_t)N * a.pitch;
+  const int i = __builtin_amdgcn_readfirstlane(blockIdx.y * a.node_block + wave);
+  if (i >= N) return;
+  const int group = __builtin_amdgcn_readfirstlane(
+      (blockIdx.x * 64 * TVBX_IPL) / a.group_size);
+  const unsigned *__restrict__ cd = a.coldelay + (size_t)group * a.nnz;
+  const tvbx_rt *__restrict__ ring = (const tvbx_rt *)a.ring;
+  tvbx_rt *ring_w = (tvbx_rt *)a.ring;
+  const size_t ip = (size_t)i * a.pitch + pb;
+
+  tvbx_lane_params lp[TVBX_IPL];
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q) tvbx_load_lane(lp[q], a, pb + q);
+  unsigned bad = 0u;
+  long long n0;
+  int t;
+  tvbx_start(a, n0, t);
+
+  float x[TVBX_IPL][TVBX_S];
+#pragma unroll
+  for (int s = 0; s < TVBX_S; ++s) {
+    float v[TVBX_IPL];
+    tvbx_ldf(a.state + s * NP + ip, v);
+#pragma unroll
+    for (int q = 0; q < TVBX_IPL; ++q) x[q][s] = v[q];
+  }
+  float acc_t[TVBX_IPL];
+  if (a.tavg_period > 0) tvbx_ldf(a.tavg + ip, acc_t);
+
+  for (int k = 0; k < a.n_steps; ++k) {
+    const long long n = n0 + k;
+    const int t1 = (t + 1 == H) ? 0 : t + 1;
+    if (a.rec_x != nullptr) tvbx_record_state(a, ip, n, x);
+    float y_mon[TVBX_IPL];
+    if (a.tavg_period > 0) tvbx_monitor_obs(a, i, pb, lp, x, y_mon);
+    float c[TVBX_IPL][TVBX_C];
+    tvbx_coupling(a, cd, ring, nullptr, i, t, pb, n, lp, c);
+    float yo[TVBX_R][TVBX_IPL];
+    tvbx_advance(a, i, pb, n, k, lp, x, c, yo, bad);
+#pragma unroll
+    for (int r = 0; r < TVBX_R; ++r)
+      tvbx_str(ring_w + tvbx_ring_off(t1, r, i, a) + pb, yo[r]);
+    if (a.tavg_period > 0) tvbx_monitor(a, ip, n, y_mon, acc_t);
+    t = t1;
+  }
+#pragma unroll
+  for (int s = 0; s < TVBX_S; ++s) {
+    float v[TVBX_IPL];
+#pragma unroll
+    for (int q = 0; q < TVBX_IPL; ++q) v[q] = x[q][s];
+    tvbx_stf(a.state + s * NP + ip, v);
+  }
+  if (a.tavg_period > 0) tvbx_stf(a.tavg + ip, acc_t);
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q)
+    if ((bad >> q) & 1u) a.bad[pb + q] = 1u;
+}
+#endif
 
 // Fill the ring from the current state: slot t = h(x); other slots 0
 // (mode 0) or h(x) (mode 1); mode 2 writes slot t only.  In the reference

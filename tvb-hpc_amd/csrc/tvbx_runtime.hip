@@ -139,6 +139,7 @@ struct tvbx_module {
   hipFunction_t step = nullptr;
   hipFunction_t hist = nullptr;
   hipFunction_t adj = nullptr;  // present when emitted with the adjoint
+  hipFunction_t blocked = nullptr;  // absent in the reference ordering
   std::vector<char> image;
 };
 
@@ -162,6 +163,7 @@ struct tvbx_sim {
   DevBuf row, coldelay, weight, state, ring, p_inst, p_node, p_ni, p_glob;
   DevBuf out_input, out_drift, out_diffs, tavg, bold_state, samples, bad,
       noise, fc_mean, fc_m2, corr, stale;
+  DevBuf fc_tiles, fc_shift, fc_s1, fc_u;  // FC merge work buffers
   // monitor bookkeeping: sample m is in slot m % sample_slots
   long long samples_made = 0;  // samples written since the monitor started
   long long samples_used = 0;  // samples folded into FC
@@ -190,6 +192,13 @@ struct tvbx_sim {
   DevBuf lam, mu, rec_x, rec_sum, forcing, g_c, g_m, g_out, save_state,
       save_ring;
 };
+
+// the forward step kernel of the simulation's launch shape
+static hipFunction_t step_fn(const tvbx_sim *s) {
+  const bool blocked =
+      s->d.steps_per_launch > 1 && s->d.node_block < s->d.n_node;
+  return blocked ? s->mod->blocked : s->mod->step;
+}
 
 // advance the device clock by w steps (last node of a captured window)
 __global__ void k_tick(long long *clock, long long w, int H) {
@@ -286,6 +295,9 @@ static int module_from_image(tvbx_ctx *c, const void *image, size_t size,
   }
   if (hipModuleGetFunction(&m->adj, m->mod, "tvbx_adjoint") != hipSuccess)
     m->adj = nullptr;
+  if (hipModuleGetFunction(&m->blocked, m->mod, "tvbx_step_blocked") !=
+      hipSuccess)
+    m->blocked = nullptr;
   (void)hipGetLastError();
   *out = m;
   return TVBX_OK;
@@ -387,82 +399,138 @@ __host__ __device__ inline size_t tri_index(int i, int j, int N) {
   return (size_t)i * N - (size_t)i * (i - 1) / 2 + (size_t)(j - i);
 }
 
-#define FC_TB 8
+#define FC_TB 8   // node tile edge of the co-moment kernel
+#define FC_ST 8   // tiles per super-tile edge (rows kept in one XCD's L2)
 
-// Chan et al. pairwise update of FP64 co-moments with a block of `count`
-// samples (slots first, first+1, ... modulo slots):
-//   M2_ij += sum_k (x_ki - mb_i)(x_kj - mb_j) + d_i d_j n_a m / (n_a + m),
-//   d = mb - ma,
-// where mb is the block mean and ma the running mean of the n_a previous
-// samples.  One lane per instance; a workgroup covers an 8 x 8 tile of node
-// pairs (only tiles with I <= J do work).
+// Merge of a block of m = `count` monitor samples (slots first, first + 1,
+// ... modulo slots) into FP64 co-moments M2 and means ma of the n_a samples
+// merged before (Chan, Golub & LeVeque's pairwise update):
+//   M2_ij += sum_k (x_ki - mb_i)(x_kj - mb_j) + u_i u_j n_a m / (n_a + m),
+//   u = mb - ma.
+// With d_k = x_k - shift for any per-(node, instance) shift,
+//   sum_k (x_ki - mb_i)(x_kj - mb_j) = sum_k d_ki d_kj - S1_i S1_j / m,
+//   S1 = sum_k d_k,
+// so the block is read once.  The shift is the running mean rounded to
+// float (the block's first sample before the first merge), which keeps d
+// small and makes x - shift exact in double.
+
+// per (node, instance): shift, S1 and u of the block
+__global__ void k_fc_prep(const float *__restrict__ samples, int slots,
+                          long long first, int count, int N, int pitch,
+                          double n_prev, const double *__restrict__ mean,
+                          float *__restrict__ shift, double *__restrict__ s1,
+                          double *__restrict__ u) {
+  const int p = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y;
+  const size_t NP = (size_t)N * pitch;
+  const size_t ip = (size_t)i * pitch + p;
+  const double ma = mean[ip];
+  const float sh = n_prev > 0.0 ? (float)ma
+                                : samples[(size_t)(first % slots) * NP + ip];
+  double s = 0.0;
+  for (int k = 0; k < count; ++k)
+    s += (double)samples[(size_t)((first + k) % slots) * NP + ip] - (double)sh;
+  shift[ip] = sh;
+  s1[ip] = s;
+  u[ip] = ((double)sh + s / (double)count) - ma;
+}
+
+// co-moments of one 8 x 8 tile of node pairs (I <= J) for the 64 instances
+// of one lane block.  Workgroups are numbered so that each of the 8 XCDs
+// (which take workgroup ids round robin) works through a contiguous range
+// of (instance block, tile), tiles in super-tile order: the sample rows of
+// a super-tile stay in that XCD's L2 while its tiles run.
 __global__ void __launch_bounds__(64)
 k_fc_update(const float *__restrict__ samples, int slots, long long first,
             int count, int N, int pitch, double n_prev,
-            const double *__restrict__ mean, double *__restrict__ m2) {
-  const int p = blockIdx.x * 64 + threadIdx.x;
-  const int T = (N + FC_TB - 1) / FC_TB;
-  const int I = blockIdx.y / T, J = blockIdx.y % T;
-  if (I > J) return;
+            const int2 *__restrict__ tiles, int n_tiles, int n_pblk,
+            const float *__restrict__ shift, const double *__restrict__ s1,
+            const double *__restrict__ u, double *__restrict__ m2) {
+  const unsigned per = gridDim.x / 8;  // grid is a multiple of 8
+  const unsigned L = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (L >= (unsigned)n_tiles * (unsigned)n_pblk) return;
+  const int pblk = L / n_tiles;
+  const int2 tj = tiles[L % n_tiles];
+  const int p = pblk * 64 + threadIdx.x;
+  const int I0 = tj.x * FC_TB, J0 = tj.y * FC_TB;
   const size_t NP = (size_t)N * pitch;
-  double mbi[FC_TB], mbj[FC_TB], m2b[FC_TB][FC_TB];
+  float shi[FC_TB], shj[FC_TB];
+#pragma unroll
   for (int a = 0; a < FC_TB; ++a) {
-    mbi[a] = 0.0;
-    mbj[a] = 0.0;
-    for (int b = 0; b < FC_TB; ++b) m2b[a][b] = 0.0;
+    const int i = I0 + a, j = J0 + a;
+    shi[a] = i < N ? shift[(size_t)i * pitch + p] : 0.0f;
+    shj[a] = j < N ? shift[(size_t)j * pitch + p] : 0.0f;
   }
-  for (int k = 0; k < count; ++k) {
-    const float *x = samples + (size_t)((first + k) % slots) * NP;
+  double c[FC_TB][FC_TB];
+#pragma unroll
+  for (int a = 0; a < FC_TB; ++a)
+#pragma unroll
+    for (int b = 0; b < FC_TB; ++b) c[a][b] = 0.0;
+  // one sample of prefetch
+  float xi[FC_TB], xj[FC_TB];
+  {
+    const float *x = samples + (size_t)(first % slots) * NP + p;
+#pragma unroll
     for (int a = 0; a < FC_TB; ++a) {
-      int i = I * FC_TB + a, j = J * FC_TB + a;
-      if (i < N) mbi[a] += (double)x[(size_t)i * pitch + p];
-      if (j < N) mbj[a] += (double)x[(size_t)j * pitch + p];
+      xi[a] = I0 + a < N ? x[(size_t)(I0 + a) * pitch] : 0.0f;
+      xj[a] = J0 + a < N ? x[(size_t)(J0 + a) * pitch] : 0.0f;
     }
   }
-  const double rc = 1.0 / (double)count;
-  for (int a = 0; a < FC_TB; ++a) {
-    mbi[a] *= rc;
-    mbj[a] *= rc;
-  }
   for (int k = 0; k < count; ++k) {
-    const float *x = samples + (size_t)((first + k) % slots) * NP;
-    double xi[FC_TB], xj[FC_TB];
-    for (int a = 0; a < FC_TB; ++a) {
-      int i = I * FC_TB + a, j = J * FC_TB + a;
-      xi[a] = i < N ? (double)x[(size_t)i * pitch + p] - mbi[a] : 0.0;
-      xj[a] = j < N ? (double)x[(size_t)j * pitch + p] - mbj[a] : 0.0;
+    float ni[FC_TB], nj[FC_TB];
+    if (k + 1 < count) {
+      const float *x = samples + (size_t)((first + k + 1) % slots) * NP + p;
+#pragma unroll
+      for (int a = 0; a < FC_TB; ++a) {
+        ni[a] = I0 + a < N ? x[(size_t)(I0 + a) * pitch] : 0.0f;
+        nj[a] = J0 + a < N ? x[(size_t)(J0 + a) * pitch] : 0.0f;
+      }
     }
-    for (int a = 0; a < FC_TB; ++a)
-      for (int b = 0; b < FC_TB; ++b) m2b[a][b] = fma(xi[a], xj[b], m2b[a][b]);
+    double dj[FC_TB];
+#pragma unroll
+    for (int b = 0; b < FC_TB; ++b) dj[b] = (double)xj[b] - (double)shj[b];
+#pragma unroll
+    for (int a = 0; a < FC_TB; ++a) {
+      const double di = (double)xi[a] - (double)shi[a];
+#pragma unroll
+      for (int b = 0; b < FC_TB; ++b) c[a][b] = fma(di, dj[b], c[a][b]);
+    }
+    if (k + 1 < count) {
+#pragma unroll
+      for (int a = 0; a < FC_TB; ++a) {
+        xi[a] = ni[a];
+        xj[a] = nj[a];
+      }
+    }
   }
-  const double w = n_prev * (double)count / (n_prev + (double)count);
+  const double m = (double)count;
+  const double w = n_prev * m / (n_prev + m);
+#pragma unroll
   for (int a = 0; a < FC_TB; ++a) {
-    const int i = I * FC_TB + a;
+    const int i = I0 + a;
     if (i >= N) continue;
-    const double di = mbi[a] - mean[(size_t)i * pitch + p];
+    const double s1i = s1[(size_t)i * pitch + p];
+    const double ui = u[(size_t)i * pitch + p];
+#pragma unroll
     for (int b = 0; b < FC_TB; ++b) {
-      const int j = J * FC_TB + b;
+      const int j = J0 + b;
       if (j >= N || j < i) continue;
-      const double dj = mbj[b] - mean[(size_t)j * pitch + p];
+      const double s1j = s1[(size_t)j * pitch + p];
+      const double uj = u[(size_t)j * pitch + p];
       double *dst = m2 + tri_index(i, j, N) * pitch + p;
-      *dst = *dst + m2b[a][b] + di * dj * w;
+      *dst = *dst + (c[a][b] - s1i * s1j / m + ui * uj * w);
     }
   }
 }
 
-// running mean update after k_fc_update: ma += (mb - ma) m / (n_a + m)
-__global__ void k_fc_mean(const float *__restrict__ samples, int slots,
-                          long long first, int count, int N, int pitch,
-                          double n_prev, double *__restrict__ mean) {
+// running mean after the merge: ma += u m / (n_a + m)
+__global__ void k_fc_mean(int N, int pitch, int count, double n_prev,
+                          const double *__restrict__ u,
+                          double *__restrict__ mean) {
   const int p = blockIdx.x * 64 + threadIdx.x;
-  const int i = blockIdx.y;
-  const size_t NP = (size_t)N * pitch;
-  double s = 0.0;
-  for (int k = 0; k < count; ++k)
-    s += (double)samples[(size_t)((first + k) % slots) * NP + (size_t)i * pitch + p];
-  const double mb = s / (double)count;
-  double *ma = mean + (size_t)i * pitch + p;
-  *ma = *ma + (mb - *ma) * ((double)count / (n_prev + (double)count));
+  const size_t ip = (size_t)blockIdx.y * pitch + p;
+  const double m = (double)count;
+  mean[ip] = mean[ip] + u[ip] * (m / (n_prev + m));
 }
 
 // corr_ij = M2_ij / sqrt(M2_ii M2_jj) (metric.py:111-139; the 1/n of the
@@ -623,8 +691,14 @@ extern "C" int tvbx_sim_create(tvbx_ctx *c, tvbx_module *m,
   CHECK(d.waves >= 1 && d.waves <= 16, "waves must be in [1, 16]");
   CHECK(d.node_block >= 1, "node_block must be positive");
   CHECK(d.steps_per_launch >= 1, "steps_per_launch must be positive");
-  CHECK(d.steps_per_launch == 1 || d.node_block >= d.n_node,
-        "multi-step launches need one workgroup per instance tile");
+  // several steps per launch: either one workgroup holds all nodes of its
+  // instance tile (barriers between steps), or temporal blocking with one
+  // node per wave, which the delays must allow (checked with the connectome)
+  const bool blocked = d.steps_per_launch > 1 && d.node_block < d.n_node;
+  CHECK(!blocked || (d.node_block == d.waves && d.waves <= 8 &&
+                     d.order == 0 && m->blocked),
+        "multi-step launches with node_block < n_node need node_block == "
+        "waves <= 8, the CUDA ordering and a module with tvbx_step_blocked");
   CHECK(d.tavg_period >= 0, "tavg_period must be >= 0");
   CHECK(d.tavg_period == 0 || d.sample_slots >= 1, "sample_slots must be >= 1");
   CHECK(!d.fc || d.tavg_period > 0, "fc needs the monitor (tavg_period > 0)");
@@ -722,7 +796,8 @@ extern "C" int tvbx_sim_destroy(tvbx_sim *s) {
                    &s->p_inst, &s->p_node, &s->p_ni, &s->p_glob,
                    &s->out_input, &s->out_drift, &s->out_diffs, &s->tavg,
                    &s->bold_state, &s->samples, &s->bad, &s->noise,
-                   &s->fc_mean, &s->fc_m2, &s->corr, &s->stale};
+                   &s->fc_mean, &s->fc_m2, &s->corr, &s->stale,
+                   &s->fc_tiles, &s->fc_shift, &s->fc_s1, &s->fc_u};
   for (DevBuf *b : all) dfree(*b);
   dfree(s->clock);
   DevBuf *adj[] = {&s->ccol, &s->crow, &s->cw, &s->lam, &s->mu, &s->rec_x,
@@ -761,6 +836,7 @@ extern "C" int tvbx_sim_set_connectome(tvbx_sim *s, const int32_t *row,
   const tvbx_sim_desc &d = s->d;
   CHECK(d.nnz == 0 || (col && weight && delays), "null argument");
   CHECK(row[0] == 0, "row[0] must be 0");
+  const bool blocked = d.steps_per_launch > 1 && d.node_block < d.n_node;
   CHECK(row[d.n_node] == d.nnz, "nnz (%d) disagrees with row[N] (%d)", d.nnz,
         row[d.n_node]);
   for (int i = 0; i < d.n_node; ++i)
@@ -776,6 +852,14 @@ extern "C" int tvbx_sim_set_connectome(tvbx_sim *s, const int32_t *row,
             "delay %d of edge %d (group %d) needs horizon >= %d (have %d)",
             dl, e, g, dl + 2, d.horizon);
       CHECK(dl < 65536, "delay %d too large", dl);
+      // temporal blocking: a launch of L steps must only read slots written
+      // before it and never the L slots it writes
+      CHECK(!blocked || (dl >= 1 && dl + 1 >= d.steps_per_launch &&
+                         dl + d.steps_per_launch + 1 <= d.horizon),
+            "delay %d of edge %d (group %d) does not allow %d steps per "
+            "launch with horizon %d (need 1 <= delay, steps <= delay + 1, "
+            "horizon >= delay + steps + 1)",
+            dl, e, g, d.steps_per_launch, d.horizon);
       cd[(size_t)g * d.nnz + e] = ((unsigned)dl << 16) | (unsigned)col[e];
     }
   for (int e = 0; e < d.nnz; ++e)
@@ -947,23 +1031,63 @@ extern "C" int tvbx_sim_set_noise(tvbx_sim *s, int mode,
   return TVBX_OK;
 }
 
+// tiles (I, J), I <= J, of the upper triangle in super-tile order
+static std::vector<int> fc_tile_order(int T) {
+  std::vector<int> t;
+  const int TS = (T + FC_ST - 1) / FC_ST;
+  for (int si = 0; si < TS; ++si)
+    for (int sj = si; sj < TS; ++sj)
+      for (int I = si * FC_ST; I < std::min(T, si * FC_ST + FC_ST); ++I)
+        for (int J = std::max(I, sj * FC_ST);
+             J < std::min(T, sj * FC_ST + FC_ST); ++J) {
+          t.push_back(I);
+          t.push_back(J);
+        }
+  return t;
+}
+
 static int fc_flush(tvbx_sim *s) {
   const tvbx_sim_desc &d = s->d;
   long long pending = s->samples_made - s->samples_used;
   if (!d.fc || pending <= 0) return TVBX_OK;
   const int N = d.n_node;
   const int T = (N + FC_TB - 1) / FC_TB;
+  const int n_tiles = T * (T + 1) / 2;
+  int rc = TVBX_OK;
+  if (!s->fc_tiles.p) {
+    std::vector<int> t = fc_tile_order(T);
+    rc = dalloc(s->fc_tiles, t.size() * 4);
+    if (rc != TVBX_OK) return rc;
+    HIPCHK(hipMemcpyAsync(s->fc_tiles.p, t.data(), t.size() * 4,
+                          hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  }
+  rc = dalloc(s->fc_shift, s->NP * 4);
+  if (rc == TVBX_OK) rc = dalloc(s->fc_s1, s->NP * 8);
+  if (rc == TVBX_OK) rc = dalloc(s->fc_u, s->NP * 8);
+  if (rc != TVBX_OK) return rc;
   const long long first = (s->sample_base + s->samples_used) % d.sample_slots;
   const double n_prev = (double)s->samples_used;
-  hipLaunchKernelGGL(k_fc_update, dim3(s->pitch / 64, T * T), dim3(64), 0,
-                     s->ctx->stream, (const float *)s->samples.p,
-                     d.sample_slots, first, (int)pending, N, s->pitch, n_prev,
-                     (const double *)s->fc_mean.p, (double *)s->fc_m2.p);
+  const int n_pblk = s->pitch / 64;
+  hipLaunchKernelGGL(k_fc_prep, dim3(n_pblk, N), dim3(64), 0, s->ctx->stream,
+                     (const float *)s->samples.p, d.sample_slots, first,
+                     (int)pending, N, s->pitch, n_prev,
+                     (const double *)s->fc_mean.p, (float *)s->fc_shift.p,
+                     (double *)s->fc_s1.p, (double *)s->fc_u.p);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_fc_mean, dim3(s->pitch / 64, N), dim3(64), 0,
+  const unsigned long long nwg =
+      ((unsigned long long)n_tiles * n_pblk + 7) / 8 * 8;
+  CHECK(nwg < (1ull << 31), "FC problem too large for one launch");
+  hipLaunchKernelGGL(k_fc_update, dim3((unsigned)nwg), dim3(64), 0,
                      s->ctx->stream, (const float *)s->samples.p,
                      d.sample_slots, first, (int)pending, N, s->pitch, n_prev,
-                     (double *)s->fc_mean.p);
+                     (const int2 *)s->fc_tiles.p, n_tiles, n_pblk,
+                     (const float *)s->fc_shift.p, (const double *)s->fc_s1.p,
+                     (const double *)s->fc_u.p, (double *)s->fc_m2.p);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_fc_mean, dim3(n_pblk, N), dim3(64), 0, s->ctx->stream,
+                     N, s->pitch, (int)pending, n_prev,
+                     (const double *)s->fc_u.p, (double *)s->fc_mean.p);
   HIPCHK(hipGetLastError());
   s->samples_used = s->samples_made;
   return TVBX_OK;
@@ -1029,7 +1153,7 @@ static int capture_window(tvbx_sim *s, long long W, hipGraphExec_t *out) {
     a.n_steps = d.steps_per_launch;
     a.clock = (const long long *)s->clock.p;
     void *params[] = {&a};
-    e = hipModuleLaunchKernel(s->mod->step, gx, gy, 1, bx, 1, 1, 0, st,
+    e = hipModuleLaunchKernel(step_fn(s), gx, gy, 1, bx, 1, 1, 0, st,
                               params, nullptr);
   }
   if (e == hipSuccess) {
@@ -1160,7 +1284,7 @@ extern "C" int tvbx_sim_run(tvbx_sim *s, long long n_steps,
       e1 = next_event(s);
       if (e0) hipEventRecord(e0, s->ctx->stream);
     }
-    HIPCHK(hipModuleLaunchKernel(s->mod->step, gx, gy, 1, bx, 1, 1, 0,
+    HIPCHK(hipModuleLaunchKernel(step_fn(s), gx, gy, 1, bx, 1, 1, 0,
                                  s->ctx->stream, params, nullptr));
     if (s->profile && e1) hipEventRecord(e1, s->ctx->stream);
     s->launches++;
@@ -1344,9 +1468,9 @@ extern "C" int tvbx_corr(tvbx_ctx *c, const float *samples, long long n,
     rc = tvbx_sim_fc(&s, 0, P, corr, nullptr);
   }
   dfree(s.samples);
-  dfree(s.fc_mean);
-  dfree(s.fc_m2);
-  dfree(s.corr);
+  DevBuf *all[] = {&s.fc_mean, &s.fc_m2, &s.corr, &s.fc_tiles, &s.fc_shift,
+                   &s.fc_s1, &s.fc_u};
+  for (DevBuf *b : all) dfree(*b);
   return rc;
 }
 
@@ -1510,6 +1634,11 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
   const unsigned gy = (N + d.node_block - 1) / d.node_block;
   const unsigned bx = 64u * d.waves;
   const int spl = d.steps_per_launch;
+  // temporal blocking in reverse: step n reads mu(n + d), written inside
+  // the launch only if d < steps per launch, so reverse launches take at
+  // most d_min >= steps_per_launch - 1 steps
+  const bool blocked = spl > 1 && d.node_block < N;
+  const int spl_rev = blocked ? std::max(1, spl - 1) : spl;
   for (long long kw = K - 1; kw >= 0; --kw) {
     HIPCHK(hipMemcpyAsync(s->state.p, s->ck_state[kw].p, s->state.bytes,
                           hipMemcpyDeviceToDevice, s->ctx->stream));
@@ -1528,17 +1657,17 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
       a.rec_sum = (float *)s->rec_sum.p;
       a.rec_step0 = n_lo;
       void *params[] = {&a};
-      HIPCHK(hipModuleLaunchKernel(s->mod->step, gx, gy, 1, bx, 1, 1, 0,
+      HIPCHK(hipModuleLaunchKernel(step_fn(s), gx, gy, 1, bx, 1, 1, 0,
                                    s->ctx->stream, params, nullptr));
     }
     // reverse sweep of the window
-    for (long long k = 0; k < W; k += spl) {
+    for (long long k = 0; k < W; k += spl_rev) {
       tvbx_adj_args A;
       memset(&A, 0, sizeof A);
       s->step = n_lo;
       fill_args(s, A.f);
       A.f.step0 = n_lo + W - 1 - k;
-      A.f.n_steps = (int)std::min<long long>(spl, W - k);
+      A.f.n_steps = (int)std::min<long long>(spl_rev, W - k);
       A.f.rec_step0 = n_lo;
       A.ccol = (const int *)s->ccol.p;
       A.crow = (const unsigned *)s->crow.p;

@@ -14,9 +14,16 @@ The choices:
   between steps); with node_block >= N one workgroup holds a whole instance
   tile and ``steps_per_launch`` steps run inside one launch, separated by
   workgroup barriers.
+* ``unroll``: delayed gathers issued together per CSR row group (8, 16,
+  32): more loads in flight per wave against more registers.
+* temporal blocking: with one node per wave (node_block == waves < N),
+  ``steps_per_launch`` up to d_min + 1 (d_min the smallest delay) steps run
+  in one launch with no synchronisation at all, the state and the monitor
+  accumulator staying in registers, because no step of the launch reads a
+  ring slot another workgroup writes in it.
 
 ``tune(cfg)`` times candidates on the device with short runs and records
-the best per (kernel source, N, P, nnz, device) in a JSON table next to the
+the best per (workload, N, P, nnz, device) in a JSON table next to the
 library; ``best_launch(cfg)`` reads that table, falling back to measured
 defaults for MI355X.
 """
@@ -24,6 +31,8 @@ defaults for MI355X.
 import json
 import os
 import time
+
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, '_lib', 'tuned.json')
@@ -56,7 +65,20 @@ def candidates(cfg):
             for waves in (8, 16):
                 out.append(dict(ipl=ipl, waves=waves, node_block=N,
                                 steps_per_launch=50))
+        for spl in block_lengths(cfg):
+            for waves in (1, 2, 4, 8):
+                if waves < N:
+                    out.append(dict(ipl=ipl, waves=waves, node_block=waves,
+                                    steps_per_launch=spl))
     return out
+
+
+def block_lengths(cfg):
+    "Temporal-blocking lengths the delays allow (up to d_min + 1, <= 32)."
+    if cfg.csr.nnz == 0:
+        return []
+    lmax = int(np.min(cfg.csr.delays)) + 1
+    return sorted({min(lmax, b) for b in (8, 16, 32)} - {1})
 
 
 def default_launch(cfg):
@@ -68,10 +90,15 @@ def default_launch(cfg):
     return dict(ipl=ipl, waves=4, node_block=4, steps_per_launch=1)
 
 
+def _prefix(cfg):
+    # the workload (name and sizes), not the kernel source: a launch shape
+    # stays a good starting point across source revisions
+    return '%s|N%d|P%d|nnz%d|' % (cfg.name, cfg.csr.nnode, cfg.n_inst,
+                                  cfg.csr.nnz)
+
+
 def _key(cfg, ctx_info):
-    return '%s|N%d|P%d|nnz%d|%s' % (cfg.spec().key(), cfg.csr.nnode,
-                                    cfg.n_inst, cfg.csr.nnz,
-                                    ctx_info.get('arch', '?'))
+    return _prefix(cfg) + ctx_info.get('arch', '?')
 
 
 def _load():
@@ -93,26 +120,47 @@ def best_launch(cfg, ctx=None):
     return default_launch(cfg)
 
 
-def tune(cfg, steps=100, warmup=10, ctx=None, log=None):
-    """Time every candidate for `steps` steps; record and return the best."""
+def table_launches(cfg):
+    "Launches recorded for this configuration (on any device)."
+    prefix = _prefix(cfg)
+    return [v['launch'] for k, v in _load().items() if k.startswith(prefix)]
+
+
+def _time(cfg, cand, x0, steps, warmup, ctx):
+    sim = cfg.simulation(context=ctx, **cand)
+    sim.set_state(x0)
+    sim.init_history('zeros')
+    sim.run(warmup)
+    sim.sync()
+    t = time.perf_counter()
+    sim.run(steps)
+    sim.sync()
+    ms = (time.perf_counter() - t) / steps * 1e3
+    del sim
+    return ms
+
+
+def tune(cfg, steps=100, warmup=10, ctx=None, log=None, unrolls=(16, 32),
+         top=3):
+    """Time every launch shape for `steps` steps, then the `top` best with
+    the other gather unrolls; record and return the best."""
     from .hip import Context
     ctx = ctx or Context.default()
     x0 = cfg.initial_state()
     results = []
     for cand in candidates(cfg):
-        sim = cfg.simulation(context=ctx, **cand)
-        sim.set_state(x0)
-        sim.init_history('zeros')
-        sim.run(warmup)
-        sim.sync()
-        t = time.perf_counter()
-        sim.run(steps)
-        sim.sync()
-        ms = (time.perf_counter() - t) / steps * 1e3
+        ms = _time(cfg, cand, x0, steps, warmup, ctx)
         results.append((ms, cand))
         if log:
             log('%s %.4f ms/step' % (cand, ms))
-        del sim
+    results.sort(key=lambda r: r[0])
+    for _, base in list(results[:top]):
+        for u in unrolls:
+            cand = dict(base, unroll=u)
+            ms = _time(cfg, cand, x0, steps, warmup, ctx)
+            results.append((ms, cand))
+            if log:
+                log('%s %.4f ms/step' % (cand, ms))
     results.sort(key=lambda r: r[0])
     best = results[0][1]
     table = _load()

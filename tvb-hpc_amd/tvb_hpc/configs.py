@@ -222,10 +222,15 @@ ALL = {'kuramoto76': kuramoto76, 'g2do84': g2do84,
 def all_specs():
     """Kernel specs the builtin configurations use, for every launch width
     the autotuner may pick (for ahead-of-time builds)."""
-    from .autotune import valid_ipls
-    specs = []
+    from .autotune import table_launches, valid_ipls
+    specs, keys = [], set()
     for fn in ALL.values():
         cfg = fn()
-        for ipl in valid_ipls(cfg):
-            specs.append(cfg.spec(ipl=ipl))
+        launches = [dict(ipl=ipl) for ipl in valid_ipls(cfg)]
+        launches += table_launches(cfg)
+        for la in launches:
+            sp = cfg.spec(ipl=la['ipl'], unroll=la.get('unroll', 8))
+            if sp.key() not in keys:
+                keys.add(sp.key())
+                specs.append(sp)
     return specs

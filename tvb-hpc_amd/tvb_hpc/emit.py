@@ -94,9 +94,13 @@ class KernelSpec:
 
     def __init__(self, model: BaseModel, cfun: BaseCoupling, scheme: EulerStep,
                  kinds=None, sigma_kind=None, ring='coupled', ring_pre=None,
-                 order='cuda', ipl=1, ring_dtype='f32', adjoint=False):
+                 order='cuda', ipl=1, ring_dtype='f32', adjoint=False,
+                 unroll=8):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.adjoint = bool(adjoint)
+        if unroll not in (4, 8, 16, 32):
+            raise ValueError('unroll must be 4, 8, 16 or 32')
+        self.unroll = int(unroll)
         if order not in ('cuda', 'reference'):
             raise ValueError('order must be cuda or reference')
         self.order = order
@@ -248,6 +252,7 @@ class KernelSpec:
                                                      'reference'))
         lines.append('#define TVBX_IPL %d' % self.ipl)
         lines.append('#define TVBX_RING_T %d' % RING_TYPES[self.ring_dtype])
+        lines.append('#define TVBX_UNROLL %d' % self.unroll)
         # lane parameters
         fields = self._lane_fields()
         lines.append('struct tvbx_lane_params {')

@@ -26,7 +26,7 @@ import numpy as np
 
 from . import emit
 from . import expr as E
-from .network import CSR
+from .network import CSR, horizon_for
 from .utils import getLogger
 
 LOG = getLogger(__name__)
@@ -255,7 +255,8 @@ class Simulation:
         inferred unless ``kinds`` says otherwise.  Unlisted constants keep
         their model / coupling defaults as literals.
     monitor: tavg_period (steps), bold (bool), fc (bool), sample_slots.
-    launch: waves, node_block, steps_per_launch (see tvb_hpc.autotune).
+    launch: ipl, waves, node_block, steps_per_launch, unroll (see
+        tvb_hpc.autotune).
     """
 
     def __init__(self, model, cfun, scheme, csr: CSR, n_inst, horizon=None,
@@ -264,7 +265,7 @@ class Simulation:
                  fc=False, sample_slots=64, waves=4, node_block=None,
                  steps_per_launch=1, compat_outputs=False, dt=None,
                  order='cuda', ipl=1, ring_dtype='f32', ring_scale=1.0,
-                 adjoint=False, context=None):
+                 adjoint=False, unroll=8, context=None):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.csr = csr
         N, P = csr.nnode, int(n_inst)
@@ -286,14 +287,32 @@ class Simulation:
         self.spec = emit.KernelSpec(model, cfun, scheme, kinds=kinds,
                                     sigma_kind=sigma_kind, ring=ring,
                                     ring_pre=ring_pre, order=order, ipl=ipl,
-                                    ring_dtype=ring_dtype, adjoint=adjoint)
+                                    ring_dtype=ring_dtype, adjoint=adjoint,
+                                    unroll=unroll)
         if ring_scale != 1.0 and not self.spec.ring_pre:
             raise ValueError('ring_scale needs a ring that holds pre(y)')
         self.values = resolved
         self.ctx = context or Context.default()
         self.module = self.ctx.module(self.spec)
-        H = int(horizon or (csr.max_delay + 2))
-        if H < csr.max_delay + 2:
+        nb, spl = int(node_block or N), int(steps_per_launch)
+        # temporal blocking: several steps per launch with the node axis
+        # split over workgroups (one node per wave), legal while every delay
+        # d >= 1 satisfies steps <= d + 1
+        self.blocked = spl > 1 and nb < N
+        if self.blocked:
+            if order == 'reference':
+                raise ValueError('temporal blocking needs the CUDA ordering')
+            if nb != int(waves) or nb > 8:
+                raise ValueError('temporal blocking takes one node per wave: '
+                                 'node_block (%d) must equal waves (%d), at '
+                                 'most 8' % (nb, waves))
+            dmin = int(np.min(csr.delays)) if csr.nnz else 0
+            if csr.nnz and (dmin < 1 or spl > dmin + 1):
+                raise ValueError('%d steps per launch need every delay >= %d '
+                                 '(minimum delay %d)' % (spl, spl - 1, dmin))
+        H_min = horizon_for(csr.max_delay, spl if self.blocked else 1)
+        H = int(horizon or H_min)
+        if H < H_min:
             raise ValueError('horizon %d too short for max delay %d' %
                              (H, csr.max_delay))
         self.H = H
@@ -323,8 +342,8 @@ class Simulation:
         d.bold, d.bold_obs, d.fc = int(bool(bold)), int(bold_obs), int(bool(fc))
         d.sample_slots = int(sample_slots)
         d.waves = int(waves)
-        d.node_block = int(node_block or N)
-        d.steps_per_launch = int(steps_per_launch)
+        d.node_block = nb
+        d.steps_per_launch = spl
         d.compat_outputs = int(bool(compat_outputs))
         d.order = 1 if order == 'reference' else 0
         d.ipl = int(ipl)

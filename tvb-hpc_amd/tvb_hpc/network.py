@@ -104,11 +104,13 @@ def bin_delays(delays, n_bins):
     return centres[idx].astype(np.int32), centres
 
 
-def horizon_for(max_delay: int) -> int:
+def horizon_for(max_delay: int, block: int = 1) -> int:
     """Ring length for a maximum delay: slot n+1 is written while slots
     n - d (0 <= d <= max_delay) are read, so the ring needs max_delay + 2
-    slots for the write never to alias a pending read."""
-    return int(max_delay) + 2
+    slots for the write never to alias a pending read.  A launch that
+    advances `block` steps without synchronisation writes slots n+1 ..
+    n+block while reading back to n - max_delay: max_delay + block + 1."""
+    return int(max_delay) + int(block) + 1
 
 
 class Network:
