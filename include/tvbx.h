@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TVBX_ABI_VERSION 1
+#define TVBX_ABI_VERSION 2
 
 #define TVBX_OK 0
 #define TVBX_E_ARG (-1)      /* invalid argument (validated before any launch) */
@@ -121,6 +121,10 @@ typedef struct tvbx_sim_desc {
   int ring_type;       /* ring element: 0 float32, 1 bfloat16, 2 float16;
                         * must match the module */
   float ring_scale;    /* reduced-precision rings store ring_scale * pre(y) */
+  int xcd_map;         /* 1: number the step kernels' workgroups so that each
+                        * of the 8 XCDs (which take workgroup ids round
+                        * robin) works on contiguous instance tiles, keeping
+                        * the ring lines that several rows read in one L2 */
 } tvbx_sim_desc;
 
 enum { TVBX_P_GLOB = 0, TVBX_P_INST = 1, TVBX_P_NODE = 2, TVBX_P_NI = 3 };

@@ -232,6 +232,28 @@ def test_temporal_blocking_matches_per_step_launches(model, cfun, dt, cparams,
     assert not simB.flags().any()
 
 
+@pytest.mark.parametrize('P', [512, 96])
+@pytest.mark.parametrize('blocked', [False, True])
+def test_xcd_block_map_matches_identity(P, blocked):
+    """The XCD-aware workgroup numbering only permutes the blocks: results
+    are bit for bit those of the identity numbering (P = 96 gives a block
+    count that is not a multiple of 8, where the numbering stays the
+    identity)."""
+    model, cfun, dt, cp = 'JansenRit', 'Sigmoidal', 0.05, {'a': 0.3}
+    L = _block_len(dt) if blocked else 1
+    out = []
+    for xm in (False, True):
+        sim, ref, noise = _pair(model, cfun, dt, cp, P=P, steps=300,
+                                launch=dict(waves=4, node_block=4,
+                                            steps_per_launch=L,
+                                            xcd_map=xm))
+        sim.run(300, noise=noise)
+        out.append(sim.get_state())
+    np.testing.assert_array_equal(out[0], out[1])
+    ref.run(300, noise=noise)
+    assert state_err(model, out[1], ref.get_state()) < TOL
+
+
 def test_temporal_blocking_monitor_samples():
     """tavg + BOLD samples and FC of blocked launches equal those of
     one-step launches."""

@@ -10,6 +10,7 @@
 //   TVBX_IPL         sweep instances per lane (1, 2 or 4)
 //   TVBX_RING_T      ring element: 0 float32, 1 bfloat16, 2 float16
 //   TVBX_UNROLL      delayed gathers issued together per CSR row group
+//   TVBX_MIN_WAVES   waves per SIMD the blocked kernel's registers allow
 //   struct tvbx_lane_params + tvbx_load_lane()   per-instance parameters
 //   tvbx_pre(), tvbx_post(), tvbx_dfun(), tvbx_obsrv(), tvbx_wrap(),
 //   tvbx_ring_value(), tvbx_bold_step()
@@ -207,33 +208,114 @@ __device__ __forceinline__ size_t tvbx_ring_off(int slot, int k, int j,
   return (size_t)((slot * TVBX_R + k) * a.n_node + j) * (size_t)a.pitch;
 }
 
-// y_j(n - d) of coupled observable kk for this lane's instances, as loaded.
-// In the reference (loopy) ordering a zero delay reads what slot n held
-// before step n rewrote it (st_cur).  d is wave-uniform: the branch is a
-// scalar one.
-__device__ __forceinline__ tvbx_raw tvbx_delayed(
-    const tvbx_rt *__restrict__ ring, const tvbx_rt *__restrict__ st_cur,
-    int t, int d, int kk, int j, int pb, const tvbx_step_args &a) {
-  int slot = t - d;
-  slot += (slot < 0) ? a.horizon : 0;
-#if TVBX_ORDER_REF
-  if (d == 0)
-    return tvbx_ldraw(st_cur + (size_t)(kk * a.n_node + j) * a.pitch + pb);
-#endif
-  return tvbx_ldraw(ring + tvbx_ring_off(slot, kk, j, a) + pb);
+// The 32-bit per-lane byte offset of a gather, made opaque to the optimiser
+// (an empty asm statement, no instruction) so that it is not folded into a
+// loop-invariant 64-bit per-lane pointer: the gather then stays a
+// global_load with the wave-uniform address in scalar registers plus this
+// offset, with no per-lane 64-bit address arithmetic.
+__device__ __forceinline__ unsigned tvbx_lane_offset(unsigned o) {
+  asm volatile("" : "+v"(o));
+  return o;
 }
 
+// y_j(n - d) of coupled observable kk for this lane's instances, as loaded;
+// lane_off = pb * sizeof(ring element).  In the reference (loopy) ordering
+// a zero delay reads what slot n held before step n rewrote it (st_cur).
+// d is wave-uniform: the branch is a scalar one.
+__device__ __forceinline__ tvbx_raw tvbx_delayed(
+    const tvbx_rt *__restrict__ ring, const tvbx_rt *__restrict__ st_cur,
+    int t, int d, int kk, int j, unsigned lane_off,
+    const tvbx_step_args &a) {
+  int slot = t - d;
+  slot += (slot < 0) ? a.horizon : 0;
+  const tvbx_rt *src = ring + tvbx_ring_off(slot, kk, j, a);
+#if TVBX_ORDER_REF
+  if (d == 0) src = st_cur + (size_t)(kk * a.n_node + j) * a.pitch;
+#endif
+  return *reinterpret_cast<const tvbx_raw *>(
+      reinterpret_cast<const char *>(src) + tvbx_lane_offset(lane_off));
+}
+
+// The CSR row of one node, held in vector registers: lane l has the packed
+// (delay, column) and weight of the row's edges l and 64 + l.  Each edge is
+// read back with v_readlane into scalar registers, so the row's indices
+// cost two vector loads instead of a chain of scalar loads per group of
+// gathers; edges past the first 128 are read with scalar loads.
+struct tvbx_row {
+  int e0, e1;        // CSR range
+  unsigned cd0, cd1; // lane l: edge e0 + l, e0 + 64 + l
+  float w0, w1;
+};
+
+__device__ __forceinline__ void tvbx_load_row(const tvbx_step_args &a,
+                                              const unsigned *__restrict__ cd,
+                                              int i, tvbx_row &r) {
+  const int lane = threadIdx.x & 63;
+  r.e0 = a.row[i];
+  r.e1 = a.row[i + 1];
+  const int n = r.e1 - r.e0;
+  r.cd0 = lane < n ? cd[r.e0 + lane] : 0u;
+  r.w0 = lane < n ? a.weight[r.e0 + lane] : 0.0f;
+  r.cd1 = lane + 64 < n ? cd[r.e0 + 64 + lane] : 0u;
+  r.w1 = lane + 64 < n ? a.weight[r.e0 + 64 + lane] : 0.0f;
+}
+
+__device__ __forceinline__ unsigned tvbx_lane_u(unsigned v, int l) {
+  return (unsigned)__builtin_amdgcn_readlane((int)v, l);
+}
+
+__device__ __forceinline__ float tvbx_lane_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// sum over edges [b, e) of the row whose packed (delay, column) and weight
+// for edge b + u come from cdw(u), into acc
+#define TVBX_GATHER_RANGE(NE, CDW, WW)                                        \
+  {                                                                         \
+    int u0 = 0;                                                             \
+    for (; u0 + TVBX_UNROLL <= (NE); u0 += TVBX_UNROLL) {                   \
+      tvbx_raw v[TVBX_UNROLL];                                              \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {             \
+        const unsigned w = CDW(u0 + u);                                     \
+        v[u] = tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,            \
+                            (int)(w & 0xffffu), lane_off, a);               \
+      }                                                                     \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {             \
+        const float wu = WW(u0 + u);                                        \
+        float f[TVBX_IPL];                                                  \
+        tvbx_unpack(v[u], f);                                               \
+        _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc[q] =       \
+            fmaf(wu, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);        \
+      }                                                                     \
+    }                                                                       \
+    for (; u0 < (NE); ++u0) {                                               \
+      const unsigned w = CDW(u0);                                           \
+      float f[TVBX_IPL];                                                    \
+      tvbx_unpack(tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,         \
+                               (int)(w & 0xffffu), lane_off, a),            \
+                  f);                                                       \
+      const float we = WW(u0);                                              \
+      _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc[q] =         \
+          fmaf(we, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);          \
+    }                                                                       \
+  }
+
 // Coupling sums c[q][kk] of node i at step n (ring slot t): the CSR row
-// reduction of network.py:115-136, with rec_sum recording for the adjoint.
+// reduction of network.py:115-136 over the edges in row order, with rec_sum
+// recording for the adjoint.
 __device__ __forceinline__ void tvbx_coupling(
     const tvbx_step_args &a, const unsigned *__restrict__ cd,
-    const tvbx_rt *__restrict__ ring, const tvbx_rt *__restrict__ st_cur,
-    int i, int t, int pb, long long n, const tvbx_lane_params (&lp)[TVBX_IPL],
-    float (&c)[TVBX_IPL][TVBX_C]) {
+    const tvbx_row &row, const tvbx_rt *__restrict__ ring,
+    const tvbx_rt *__restrict__ st_cur, int i, int t, int pb, long long n,
+    const tvbx_lane_params (&lp)[TVBX_IPL], float (&c)[TVBX_IPL][TVBX_C]) {
   const size_t NP = (size_t)a.n_node * a.pitch;
   const size_t ip = (size_t)i * a.pitch + pb;
   const float *__restrict__ wt = a.weight;
-  const int e0 = a.row[i], e1 = a.row[i + 1];
+  const unsigned lane_off = (unsigned)pb * (unsigned)sizeof(tvbx_rt);
+  const int deg = row.e1 - row.e0;
+  const int n0 = min(deg, 64);
+  const int n1 = min(max(deg - 64, 0), 64);
+  const int eb = __builtin_amdgcn_readfirstlane(row.e0 + 128);
 #pragma unroll
   for (int kk = 0; kk < TVBX_C; ++kk) {
     float post[TVBX_IPL], acc[TVBX_IPL];
@@ -249,36 +331,32 @@ __device__ __forceinline__ void tvbx_coupling(
       tvbx_ldr(ring + tvbx_ring_off(t, kk, i, a) + pb, post);
 #endif
     }
-    int e = e0;
-    for (; e + TVBX_UNROLL <= e1; e += TVBX_UNROLL) {
-      tvbx_raw v[TVBX_UNROLL];
-#pragma unroll
-      for (int u = 0; u < TVBX_UNROLL; ++u) {
-        const unsigned w = cd[e + u];
-        v[u] = tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,
-                            (int)(w & 0xffffu), pb, a);
-      }
-#pragma unroll
-      for (int u = 0; u < TVBX_UNROLL; ++u) {
-        const float wu = wt[e + u];
-        float f[TVBX_IPL];
-        tvbx_unpack(v[u], f);
-#pragma unroll
-        for (int q = 0; q < TVBX_IPL; ++q)
-          acc[q] = fmaf(wu, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);
-      }
-    }
-    for (; e < e1; ++e) {
-      const unsigned w = cd[e];
+#define TVBX_CD0(x) tvbx_lane_u(row.cd0, (x))
+#define TVBX_W0(x) tvbx_lane_f(row.w0, (x))
+#define TVBX_CD1(x) tvbx_lane_u(row.cd1, (x))
+#define TVBX_W1(x) tvbx_lane_f(row.w1, (x))
+#define TVBX_CDS(x) cd[eb + (x)]
+#define TVBX_WS(x) wt[eb + (x)]
+    TVBX_GATHER_RANGE(n0, TVBX_CD0, TVBX_W0)
+    if (n1 > 0) TVBX_GATHER_RANGE(n1, TVBX_CD1, TVBX_W1)
+    // rows longer than 128 edges (rare): one edge at a time, scalar loads
+    for (int u0 = 0; u0 < deg - 128; ++u0) {
+      const unsigned w = TVBX_CDS(u0);
       float f[TVBX_IPL];
       tvbx_unpack(tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,
-                               (int)(w & 0xffffu), pb, a),
+                               (int)(w & 0xffffu), lane_off, a),
                   f);
-      const float we = wt[e];
+      const float we = TVBX_WS(u0);
 #pragma unroll
       for (int q = 0; q < TVBX_IPL; ++q)
         acc[q] = fmaf(we, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);
     }
+#undef TVBX_CD0
+#undef TVBX_W0
+#undef TVBX_CD1
+#undef TVBX_W1
+#undef TVBX_CDS
+#undef TVBX_WS
 #pragma unroll
     for (int q = 0; q < TVBX_IPL; ++q) {
       acc[q] = acc[q] * a.ring_unscale;
@@ -304,6 +382,12 @@ __device__ __forceinline__ void tvbx_advance(
   for (int q = 0; q < TVBX_IPL; ++q) {
     const int p = pb + q;
     float f[TVBX_S], g[TVBX_S], y[TVBX_NOBS];
+#if TVBX_IPL > 1
+    // one instance after the other: an empty asm statement makes this
+    // instance's state depend on the previous instance's result, so the
+    // dfun temporaries of the lane's instances are not all live at once
+    if (q > 0) asm volatile("" : "+v"(x[q][0]) : "v"(x[q - 1][0]));
+#endif
     tvbx_dfun(x[q], c[q], f, g, lp[q], a, i, p);
 #if TVBX_EM
     float xi[TVBX_S];
@@ -407,6 +491,24 @@ __device__ __forceinline__ void tvbx_record_state(
   }
 }
 
+// (instance tile, node block) of this workgroup.  With xcd_map the ids,
+// which the hardware deals out to the 8 XCDs round robin, are renumbered so
+// that XCD k runs logical blocks [k T / 8, (k + 1) T / 8), instance tile
+// outermost: the rows of one instance tile share an L2, and with it the
+// ring lines that several of them read in a step.
+__device__ __forceinline__ void tvbx_block(const tvbx_step_args &a, int &bx,
+                                           int &by) {
+  bx = blockIdx.x;
+  by = blockIdx.y;
+  if (a.xcd_map) {
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned w = blockIdx.x + gx * blockIdx.y;
+    const unsigned L = (w & 7u) * (gx * gy / 8u) + (w >> 3);
+    bx = L / gy;
+    by = L % gy;
+  }
+}
+
 // step and ring slot of the launch's first step (graph replays add the
 // device clock)
 __device__ __forceinline__ void tvbx_start(const tvbx_step_args &a,
@@ -424,17 +526,19 @@ tvbx_step(const tvbx_step_args a) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwave = blockDim.x >> 6;
-  const int pb = (blockIdx.x * 64 + lane) * TVBX_IPL;
+  int bx, by;
+  tvbx_block(a, bx, by);
+  const int pb = (bx * 64 + lane) * TVBX_IPL;
   const int N = a.n_node;
   const int H = a.horizon;
   const size_t NP = (size_t)N * a.pitch;
   // delay group of this wavefront's instances (group_size % (64 IPL) == 0)
   const int group = __builtin_amdgcn_readfirstlane(
-      (blockIdx.x * 64 * TVBX_IPL) / a.group_size);
+      (bx * 64 * TVBX_IPL) / a.group_size);
   const unsigned *__restrict__ cd = a.coldelay + (size_t)group * a.nnz;
   const tvbx_rt *__restrict__ ring = (const tvbx_rt *)a.ring;
   tvbx_rt *ring_w = (tvbx_rt *)a.ring;
-  const int i_begin = blockIdx.y * a.node_block;
+  const int i_begin = by * a.node_block;
   const int i_end = min(N, i_begin + a.node_block);
 
   tvbx_lane_params lp[TVBX_IPL];
@@ -470,7 +574,9 @@ tvbx_step(const tvbx_step_args a) {
       float y_mon[TVBX_IPL];
       if (a.tavg_period > 0) tvbx_monitor_obs(a, i, pb, lp, x, y_mon);
 
-      tvbx_coupling(a, cd, ring, st_cur, i, t, pb, n, lp, c);
+      tvbx_row rw;
+      tvbx_load_row(a, cd, i, rw);
+      tvbx_coupling(a, cd, rw, ring, st_cur, i, t, pb, n, lp, c);
 
       float yo[TVBX_R][TVBX_IPL];
       tvbx_advance(a, i, pb, n, k, lp, x, c, yo, bad);
@@ -524,18 +630,20 @@ tvbx_step(const tvbx_step_args a) {
 // waves) and keeps its state and tavg accumulator in registers for all the
 // launch's steps.  At most 8 waves per workgroup (the state of 4 instances
 // per lane needs more than 128 registers).
-extern "C" __global__ void __launch_bounds__(512)
+extern "C" __global__ void __launch_bounds__(512, TVBX_MIN_WAVES)
 tvbx_step_blocked(const tvbx_step_args a) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int pb = (blockIdx.x * 64 + lane) * TVBX_IPL;
+  int bx, by;
+  tvbx_block(a, bx, by);
+  const int pb = (bx * 64 + lane) * TVBX_IPL;
   const int N = a.n_node;
   const int H = a.horizon;
   const size_t NP = (size_t)N * a.pitch;
-  const int i = __builtin_amdgcn_readfirstlane(blockIdx.y * a.node_block + wave);
+  const int i = __builtin_amdgcn_readfirstlane(by * a.node_block + wave);
   if (i >= N) return;
   const int group = __builtin_amdgcn_readfirstlane(
-      (blockIdx.x * 64 * TVBX_IPL) / a.group_size);
+      (bx * 64 * TVBX_IPL) / a.group_size);
   const unsigned *__restrict__ cd = a.coldelay + (size_t)group * a.nnz;
   const tvbx_rt *__restrict__ ring = (const tvbx_rt *)a.ring;
   tvbx_rt *ring_w = (tvbx_rt *)a.ring;
@@ -559,6 +667,9 @@ tvbx_step_blocked(const tvbx_step_args a) {
   }
   float acc_t[TVBX_IPL];
   if (a.tavg_period > 0) tvbx_ldf(a.tavg + ip, acc_t);
+  // the node's row serves every step of the launch
+  tvbx_row rw;
+  tvbx_load_row(a, cd, i, rw);
 
   for (int k = 0; k < a.n_steps; ++k) {
     const long long n = n0 + k;
@@ -567,7 +678,7 @@ tvbx_step_blocked(const tvbx_step_args a) {
     float y_mon[TVBX_IPL];
     if (a.tavg_period > 0) tvbx_monitor_obs(a, i, pb, lp, x, y_mon);
     float c[TVBX_IPL][TVBX_C];
-    tvbx_coupling(a, cd, ring, nullptr, i, t, pb, n, lp, c);
+    tvbx_coupling(a, cd, rw, ring, nullptr, i, t, pb, n, lp, c);
     float yo[TVBX_R][TVBX_IPL];
     tvbx_advance(a, i, pb, n, k, lp, x, c, yo, bad);
 #pragma unroll

@@ -91,6 +91,9 @@ typedef struct tvbx_step_args {
   float *rec_x;
   float *rec_sum;
   long long rec_step0;
+  /* 1: remap workgroup ids so each XCD takes contiguous instance tiles
+   * (grid.x * grid.y is then a multiple of 8) */
+  int xcd_map;
 } tvbx_step_args;
 
 /* Argument block of the adjoint (reverse) kernel.  `f` carries the forward

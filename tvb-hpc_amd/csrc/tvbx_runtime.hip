@@ -664,6 +664,11 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
   a.bad = (unsigned *)s->bad.p;
   a.stale = s->stale.p;
   a.ring_scale = d.ring_type == 0 ? 1.0f : d.ring_scale;
+  {
+    const long long gx = s->pitch / (64 * d.ipl);
+    const long long gy = (d.n_node + d.node_block - 1) / d.node_block;
+    a.xcd_map = d.xcd_map && (gx * gy) % 8 == 0;
+  }
   a.ring_unscale = d.ring_type == 0 ? 1.0f : 1.0f / d.ring_scale;
 }
 
@@ -705,6 +710,7 @@ extern "C" int tvbx_sim_create(tvbx_ctx *c, tvbx_module *m,
   CHECK(d.order == 0 || d.order == 1, "order must be 0 or 1");
   CHECK(d.ipl == 1 || d.ipl == 2 || d.ipl == 4, "ipl must be 1, 2 or 4");
   CHECK(d.ring_type >= 0 && d.ring_type <= 2, "ring_type must be 0, 1 or 2");
+  CHECK(d.xcd_map == 0 || d.xcd_map == 1, "xcd_map must be 0 or 1");
   CHECK(d.ring_type == 0 || (std::isfinite(d.ring_scale) && d.ring_scale > 0),
         "ring_scale must be finite and positive");
   CHECK(d.group_size % (64 * d.ipl) == 0,

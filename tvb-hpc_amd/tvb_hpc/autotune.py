@@ -16,6 +16,10 @@ The choices:
   workgroup barriers.
 * ``unroll``: delayed gathers issued together per CSR row group (8, 16,
   32): more loads in flight per wave against more registers.
+* ``min_waves``: waves per SIMD the blocked kernel's register budget must
+  allow (more resident waves against register spills).
+* ``xcd_map``: workgroup numbering that gives each of the 8 XCDs whole
+  instance tiles, so the rows that read the same ring line share an L2.
 * temporal blocking: with one node per wave (node_block == waves < N),
   ``steps_per_launch`` up to d_min + 1 (d_min the smallest delay) steps run
   in one launch with no synchronisation at all, the state and the monitor
@@ -140,10 +144,12 @@ def _time(cfg, cand, x0, steps, warmup, ctx):
     return ms
 
 
-def tune(cfg, steps=100, warmup=10, ctx=None, log=None, unrolls=(16, 32),
-         top=3):
+def tune(cfg, steps=100, warmup=10, ctx=None, log=None,
+         variants=(dict(unroll=16), dict(unroll=4), dict(min_waves=4),
+                   dict(min_waves=4, unroll=4), dict(xcd_map=True),
+                   dict(xcd_map=True, min_waves=4)), top=3):
     """Time every launch shape for `steps` steps, then the `top` best with
-    the other gather unrolls; record and return the best."""
+    the kernel variants; record and return the best."""
     from .hip import Context
     ctx = ctx or Context.default()
     x0 = cfg.initial_state()
@@ -155,8 +161,12 @@ def tune(cfg, steps=100, warmup=10, ctx=None, log=None, unrolls=(16, 32),
             log('%s %.4f ms/step' % (cand, ms))
     results.sort(key=lambda r: r[0])
     for _, base in list(results[:top]):
-        for u in unrolls:
-            cand = dict(base, unroll=u)
+        for var in variants:
+            if 'min_waves' in var and not (base['steps_per_launch'] > 1 and
+                                           base['node_block'] <
+                                           cfg.csr.nnode):
+                continue   # only the blocked kernel takes the bound
+            cand = dict(base, **var)
             ms = _time(cfg, cand, x0, steps, warmup, ctx)
             results.append((ms, cand))
             if log:

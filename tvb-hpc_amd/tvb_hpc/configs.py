@@ -229,7 +229,8 @@ def all_specs():
         launches = [dict(ipl=ipl) for ipl in valid_ipls(cfg)]
         launches += table_launches(cfg)
         for la in launches:
-            sp = cfg.spec(ipl=la['ipl'], unroll=la.get('unroll', 8))
+            sp = cfg.spec(ipl=la['ipl'], unroll=la.get('unroll', 8),
+                          min_waves=la.get('min_waves', 1))
             if sp.key() not in keys:
                 keys.add(sp.key())
                 specs.append(sp)

@@ -95,12 +95,16 @@ class KernelSpec:
     def __init__(self, model: BaseModel, cfun: BaseCoupling, scheme: EulerStep,
                  kinds=None, sigma_kind=None, ring='coupled', ring_pre=None,
                  order='cuda', ipl=1, ring_dtype='f32', adjoint=False,
-                 unroll=8):
+                 unroll=8, min_waves=1):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.adjoint = bool(adjoint)
         if unroll not in (4, 8, 16, 32):
             raise ValueError('unroll must be 4, 8, 16 or 32')
         self.unroll = int(unroll)
+        # waves per SIMD the blocked kernel's register budget must allow
+        if min_waves not in (1, 2, 3, 4, 5, 6, 8):
+            raise ValueError('min_waves must be 1 .. 8')
+        self.min_waves = int(min_waves)
         if order not in ('cuda', 'reference'):
             raise ValueError('order must be cuda or reference')
         self.order = order
@@ -253,6 +257,7 @@ class KernelSpec:
         lines.append('#define TVBX_IPL %d' % self.ipl)
         lines.append('#define TVBX_RING_T %d' % RING_TYPES[self.ring_dtype])
         lines.append('#define TVBX_UNROLL %d' % self.unroll)
+        lines.append('#define TVBX_MIN_WAVES %d' % self.min_waves)
         # lane parameters
         fields = self._lane_fields()
         lines.append('struct tvbx_lane_params {')

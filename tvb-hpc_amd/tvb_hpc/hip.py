@@ -47,7 +47,8 @@ class SimDesc(ctypes.Structure):
         [('dt', ctypes.c_float)] + [(n, ctypes.c_int) for n in (
             'scheme', 'tavg_period', 'bold', 'bold_obs', 'fc', 'sample_slots',
             'waves', 'node_block', 'steps_per_launch', 'compat_outputs',
-            'order', 'ipl', 'ring_type')] + [('ring_scale', ctypes.c_float)]
+            'order', 'ipl', 'ring_type')] + [('ring_scale', ctypes.c_float),
+                                             ('xcd_map', ctypes.c_int)]
 
 
 _P = ctypes.c_void_p
@@ -110,7 +111,7 @@ def lib():
                 fn = getattr(L, name)
                 fn.argtypes = args
                 fn.restype = ctypes.c_int
-            if L.tvbx_abi_version() != 1:
+            if L.tvbx_abi_version() != 2:
                 raise TvbxError('libtvbx ABI version mismatch')
             _lib = L
     return _lib
@@ -255,8 +256,8 @@ class Simulation:
         inferred unless ``kinds`` says otherwise.  Unlisted constants keep
         their model / coupling defaults as literals.
     monitor: tavg_period (steps), bold (bool), fc (bool), sample_slots.
-    launch: ipl, waves, node_block, steps_per_launch, unroll (see
-        tvb_hpc.autotune).
+    launch: ipl, waves, node_block, steps_per_launch, unroll, min_waves,
+        xcd_map (see tvb_hpc.autotune).
     """
 
     def __init__(self, model, cfun, scheme, csr: CSR, n_inst, horizon=None,
@@ -265,7 +266,8 @@ class Simulation:
                  fc=False, sample_slots=64, waves=4, node_block=None,
                  steps_per_launch=1, compat_outputs=False, dt=None,
                  order='cuda', ipl=1, ring_dtype='f32', ring_scale=1.0,
-                 adjoint=False, unroll=8, context=None):
+                 adjoint=False, unroll=8, min_waves=1, xcd_map=False,
+                 context=None):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.csr = csr
         N, P = csr.nnode, int(n_inst)
@@ -288,7 +290,7 @@ class Simulation:
                                     sigma_kind=sigma_kind, ring=ring,
                                     ring_pre=ring_pre, order=order, ipl=ipl,
                                     ring_dtype=ring_dtype, adjoint=adjoint,
-                                    unroll=unroll)
+                                    unroll=unroll, min_waves=min_waves)
         if ring_scale != 1.0 and not self.spec.ring_pre:
             raise ValueError('ring_scale needs a ring that holds pre(y)')
         self.values = resolved
@@ -349,6 +351,7 @@ class Simulation:
         d.ipl = int(ipl)
         d.ring_type = emit.RING_TYPES[ring_dtype]
         d.ring_scale = float(ring_scale)
+        d.xcd_map = int(bool(xcd_map))
         self.desc = d
         self.h = ctypes.c_void_p()
         check(lib().tvbx_sim_create(self.ctx.h, self.module, ctypes.byref(d),
