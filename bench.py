@@ -2,23 +2,35 @@
 Benchmark of the delayed-coupling network integrator on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                    [--no-cpu] [--path plain]
+                    [--plain NAME|none] [--no-grad] [--no-cpu]
 
 A "step" is one integration step of the whole sweep batch (all P instances):
 delayed CSR coupling + model dfun + Euler(-Maruyama) update + history write
-(+ monitor / FC work when the workload has it).  Inputs are resident in HBM
-before the timed region; the timed region is K steps bracketed by stream
++ tavg monitor + FP64 online FC.  Inputs are resident in HBM before the
+timed region; the timed region is K steps bracketed by stream
 synchronisations.  Prints one JSON line.
 
-value        instance*steps/s of the whole batch (P * K / wall time)
+value        instance*steps/s of the batched mixed-precision sweep at N=998
+             (Jansen-Rit, P=1024, 32 delay bins, bf16 history, FP32 state,
+             FP64 FC): P * K / wall time
+plain        the same sweep through the plain FP32 kernel (exact delays,
+             float32 history); vs_plain = value / plain value
+fc_err_vs_fp64
+             max |FC_device - FC_fp64| over the sampled instances, FC of all
+             W + K steps, for the mixed and the plain path, against the
+             float64 oracle run on the host (exact delays)
+grad         gradient evaluations/s of the FC-fit inversion workload (MPR
+             N=84, P=4096, 20000 steps, checkpoints every 1000): checkpointed
+             forward + adjoint reverse sweep for all instances
 roofline     the step kernel against HBM bandwidth: algorithmic bytes per
-             launch (4 B per edge gather + 4 B per node write, per instance
-             and step) / mean launch duration (HIP events on the kernel's
-             stream); `traffic` is the measured HBM bytes per launch from the
-             committed rocprofv3 PMC summary for this kernel, if present
+             launch (2 or 4 B per edge gather + per node write, per instance
+             and step, times the steps of the launch) / mean launch duration
+             (HIP events on the simulation's stream); `traffic` is the
+             measured HBM bytes per launch from the committed rocprofv3 PMC
+             summary for this kernel, workload and launch shape
 cpu_baseline the float32 CPU oracle (the reference's loop structure: per
              instance, sequential steps, CSR row reduction, OpenMP over
-             instances) timed on a bounded sample of the same workload
+             instances) timed on a bounded sample of the plain workload
 """
 
 import argparse
@@ -40,47 +52,163 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profile(kernel, workload):
+def step_kernel(sim):
+    return 'tvbx_step_blocked' if sim.blocked else 'tvbx_step'
+
+
+def traffic_from_profile(kernel, workload, launch):
     """HBM bytes per dispatch of `kernel` on `workload` from the committed
-    PMC summary (profiles/<kernel>_<workload>_pmc.txt), or None."""
+    PMC summary (profiles/<kernel>_<workload>_pmc.txt), if it was measured
+    with this launch shape."""
     path = os.path.join(ROOT, 'profiles', '%s_%s_pmc.txt' % (kernel, workload))
     if not os.path.exists(path):
         return None
     txt = open(path).read()
     m = re.search(r'hbm_bytes_per_dispatch\s*=\s*([0-9.eE+]+)', txt)
-    return float(m.group(1)) if m else None
+    la = re.search(r'^launch\s*=\s*(\{.*\})\s*$', txt, re.M)
+    if not m or not la or json.loads(la.group(1)) != launch:
+        return None
+    return float(m.group(1))
 
 
-def cpu_baseline(cfg, x0, n_inst, n_steps, threads):
+def time_sweep(cfg, launch, K, W, graph_window, profile_steps, fc_idx):
+    """Run W + K steps (K timed), read the FC of instances fc_idx, then time
+    the step launches one by one with events.  Returns a result dict."""
+    import numpy as np
+    sim = cfg.simulation(**launch)
+    N, P, nnz = cfg.csr.nnode, cfg.n_inst, cfg.csr.nnz
+    sim.set_state(cfg.initial_state())
+    sim.init_history('zeros')
+    if getattr(cfg.scheme, 'sigma', None) is not None:
+        sim.set_noise(seed=cfg.seed + 1)
+    sim.set_exec(graph=graph_window > 0, window=max(graph_window, 1),
+                 profile=False)
+    sim.run(W)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(K)
+    sim.sync()
+    el = time.perf_counter() - t0
+    assert not sim.flags().any(), 'non-finite state in the benchmark run'
+    fc = None
+    if cfg.fc and len(fc_idx):
+        fc = np.stack([sim.fc(int(p), 1)[0] for p in fc_idx])
+    # kernel time: direct launches bracketed by events on the simulation's
+    # stream (the timed region above replays graph windows)
+    sim.set_exec(graph=False, profile=True)
+    sim.timing(reset=True)
+    sim.run(profile_steps)
+    sim.sync()
+    ms_kern, launches = sim.timing(reset=True)
+    esz = 4 if cfg.ring_dtype == 'f32' else 2
+    bytes_inst_step = esz * (nnz + N * sim.spec.C)
+    steps_per_launch = profile_steps / max(launches, 1)
+    launch_ms = ms_kern / max(launches, 1)
+    achieved = bytes_inst_step * P * steps_per_launch / (launch_ms * 1e-3) / 1e9
+    kernel = step_kernel(sim)
+    del sim
+    return dict(value=P * K / el, ms_per_step=el / K * 1e3, fc=fc,
+                roofline={
+                    'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+                    'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                    'traffic': None, 'kernel': kernel,
+                    'bytes_per_instance_step': bytes_inst_step,
+                    'steps_per_launch': steps_per_launch,
+                    'launch_ms': launch_ms, 'launches': launches})
+
+
+def time_gradient(n_evals=2, window=1000):
+    """Gradient evaluations/s of the FC-fit workload: checkpointed forward
+    of all instances, then the adjoint given dL/dsamples."""
+    import numpy as np
+    from tvb_hpc import autotune, configs
+    cfg = configs.ALL['mpr84']()
+    la = dict(autotune.best_launch(cfg))
+    T, K = cfg.n_steps, cfg.tavg_period
+    sim = cfg.simulation(adjoint=True, fc=False, sample_slots=T // K, **la)
+    x0 = cfg.initial_state()
+    dS = np.random.default_rng(0).standard_normal(
+        (T // K, cfg.csr.nnode, cfg.n_inst)).astype(np.float32) * 1e-3
+    times = []
+    for e in range(n_evals + 1):
+        sim.set_state(x0)
+        sim.step = 0
+        sim.init_history('zeros')
+        sim.set_noise(seed=5)
+        sim.sync()
+        t = time.perf_counter()
+        sim.checkpoint_run(T, window)
+        g = sim.adjoint(dS, 0)
+        sim.sync()
+        times.append(time.perf_counter() - t)
+        assert all(np.isfinite(v).all() for v in g.values())
+    ms = float(np.median(times[1:])) * 1e3
+    del sim
+    return {'workload': '%s: %s, %d steps, checkpoint every %d' %
+                        (cfg.name, cfg.description, T, window),
+            'evals_per_s': 1e3 / ms, 'ms_per_eval': ms,
+            'instance_gradients_per_s': cfg.n_inst * 1e3 / ms,
+            'params': sorted(g), 'launch': la,
+            'note': 'one evaluation = forward with checkpoints + replay + '
+                    'reverse sweep for all instances, given dL/dsamples'}
+
+
+def cpu_legs(plain_cfg, fc_idx, n_fc_steps, n_inst, n_steps, threads):
+    """The oracle on the host: (1) float32 on a bounded sample of the plain
+    workload, timed -- the CPU baseline; (2) float64 over the benchmark's
+    W + K steps for instances fc_idx -- the reference FC."""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import numpy as np
     import oracle as O
-    from tvb_hpc import configs
-    sl = cfg.sliced(n_inst)
-    csr = sl.csr
-    name = type(sl.model).__name__
-    cname = type(sl.cfun).__name__
-    mp, cp = {}, {}
-    for k, v in sl.params.items():
-        if k == 'sigma':
-            continue
-        scope, nm = k.split(':', 1) if ':' in k else ('m', k)
-        (mp if scope == 'm' else cp)[nm] = v
-    cp.update({k: v for k, v in sl.cfun.param.items()
-               if k not in cp and not hasattr(v, 'name')})
-    sigma = getattr(sl.scheme, 'sigma', None)
-    ref = O.OracleSim(name, cname, csr.nnode, csr.row, csr.col, csr.weight,
-                      csr.delays, n_inst, csr.max_delay + 2, sl.scheme.dt,
-                      scheme='em' if sigma is not None else 'euler',
-                      params=mp, cparams=cp, sigma=sigma, dtype=np.float32,
-                      n_threads=threads)
-    ref.set_state(x0[..., :n_inst])
-    ref.init_history('zeros')
-    ref.run(5, seed=1 if sigma is not None else None)
+
+    def oracle(cfg, idx, dtype):
+        csr = cfg.csr
+        mp, cp = {}, {}
+        for k, v in cfg.params.items():
+            scope, nm = k.split(':', 1)
+            a = np.asarray(v)
+            val = a[..., idx] if a.ndim >= 1 and a.shape[-1] == cfg.n_inst \
+                else a
+            (mp if scope == 'm' else cp)[nm] = val
+        cp.update({k: v for k, v in cfg.cfun.param.items()
+                   if k not in cp and not hasattr(v, 'name')})
+        ref = O.OracleSim(type(cfg.model).__name__, type(cfg.cfun).__name__,
+                          csr.nnode, csr.row, csr.col, csr.weight, csr.delays,
+                          len(idx), csr.max_delay + 2, cfg.scheme.dt,
+                          scheme='euler', params=mp, cparams=cp, dtype=dtype,
+                          tavg_period=cfg.tavg_period, n_threads=threads)
+        ref.set_state(cfg.initial_state()[..., idx])
+        ref.init_history('zeros')
+        return ref
+
+    idx = np.arange(n_inst)
+    ref = oracle(plain_cfg, idx, np.float32)
+    ref.run(5)
     t = time.perf_counter()
-    ref.run(n_steps, seed=1 if sigma is not None else None)
+    ref.run(n_steps)
     el = time.perf_counter() - t
-    return n_inst * n_steps / el, el
+    base = {'value': n_inst * n_steps / el, 'unit': 'instance*steps/s',
+            'cores': threads, 'kind': 'port',
+            'sample': '%d instances x %d steps of %s on %d threads (float32 '
+                      'oracle, %.1f s)' % (n_inst, n_steps, plain_cfg.name,
+                                           threads, el)}
+    fc64 = None
+    if len(fc_idx):
+        ref = oracle(plain_cfg, np.asarray(fc_idx), np.float64)
+        t = time.perf_counter()
+        s = ref.run(n_fc_steps)['samples']
+        log('float64 oracle FC reference: %d instances x %d steps, %.1f s' %
+            (len(fc_idx), n_fc_steps, time.perf_counter() - t))
+        fc64 = np.stack([np.corrcoef(s[:, :, k].T) for k in range(len(fc_idx))])
+    return base, fc64
+
+
+def launch_of(cfg, arg):
+    from tvb_hpc import autotune
+    if arg:
+        ipl, w, nb, spl = [int(v) for v in arg.split(',')]
+        return dict(ipl=ipl, waves=w, node_block=nb, steps_per_launch=spl)
+    return dict(autotune.best_launch(cfg))
 
 
 def main():
@@ -88,17 +216,22 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=None)
     ap.add_argument('--warmup', type=int, default=200)
-    ap.add_argument('--workload', default='jansenrit998')
-    ap.add_argument('--path', default='plain')
+    ap.add_argument('--workload', default='jansenrit998_bf16')
+    ap.add_argument('--plain', default='jansenrit998',
+                    help="plain-kernel workload to compare with, or 'none'")
+    ap.add_argument('--no-grad', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--cpu-steps', type=int, default=400)
-    ap.add_argument('--graph-window', type=int, default=100)
-    ap.add_argument('--profile-steps', type=int, default=300)
+    ap.add_argument('--fc-instances', type=int, default=8,
+                    help='instances whose FC is checked against float64')
+    ap.add_argument('--cpu-inst', type=int, default=64)
+    ap.add_argument('--cpu-steps', type=int, default=1000)
+    ap.add_argument('--graph-window', type=int, default=100,
+                    help='steps per captured graph window')
+    ap.add_argument('--no-graph', action='store_true')
+    ap.add_argument('--profile-steps', type=int, default=320)
     ap.add_argument('--launch', default=None,
                     help='ipl,waves,node_block,steps_per_launch '
                          '(default: tuned)')
-    ap.add_argument('--no-graph', action='store_true',
-                    help='time direct launches instead of graph windows')
     ap.add_argument('--copy-calibration', type=int, default=0,
                     metavar='MIB',
                     help='only run 10 elementwise copies of MIB MiB (the '
@@ -122,92 +255,74 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    from tvb_hpc import autotune, configs, hip
+    from tvb_hpc import configs
 
+    gw = 0 if args.no_graph else args.graph_window
     cfg = configs.ALL[args.workload]()
     K = args.steps or cfg.n_steps
     W = args.warmup
     N, P, nnz = cfg.csr.nnode, cfg.n_inst, cfg.csr.nnz
-    x0 = cfg.initial_state()
-    if args.launch:
-        ipl, w, nb, spl = [int(v) for v in args.launch.split(',')]
-        launch = dict(ipl=ipl, waves=w, node_block=nb, steps_per_launch=spl)
-    else:
-        launch = autotune.best_launch(cfg)
-    log('workload', cfg.name, 'N', N, 'P', P, 'nnz', nnz, 'H', cfg.horizon,
-        'launch', launch)
-    sim = cfg.simulation(**launch)
-    sim.set_state(x0)
-    sim.init_history('zeros')
-    sigma = getattr(cfg.scheme, 'sigma', None)
-    if sigma is not None:
-        sim.set_noise(seed=cfg.seed + 1)
-    sim.set_exec(graph=not args.no_graph, window=args.graph_window,
-                 profile=False)
-    sim.run(W)
-    sim.sync()
-    t0 = time.perf_counter()
-    sim.run(K)
-    sim.sync()
-    el = time.perf_counter() - t0
-    value = P * K / el
-    assert not sim.flags().any(), 'non-finite state in the benchmark run'
-    # kernel time for the roofline: direct launches bracketed by events on
-    # the simulation's stream (the timed region above uses graph windows)
-    sim.set_exec(graph=False, profile=True)
-    sim.timing(reset=True)
-    sim.run(args.profile_steps)
-    sim.sync()
-    ms_kern, launches = sim.timing(reset=True)
-    K_prof = args.profile_steps
-
-    C = sim.spec.C
-    esz = 4 if cfg.ring_dtype == 'f32' else 2
-    bytes_inst_step = esz * (nnz + N * C)
-    steps_per_launch = K_prof / max(launches, 1)
-    achieved = bytes_inst_step * P * steps_per_launch / (
-        ms_kern / launches * 1e-3) / 1e9
-    traffic = traffic_from_profile('tvbx_step', args.workload)
+    fc_idx = np.linspace(0, P - 1, args.fc_instances).astype(int) \
+        if cfg.fc and args.fc_instances > 0 else np.zeros(0, int)
+    launch = launch_of(cfg, args.launch)
+    log('workload', cfg.name, 'N', N, 'P', P, 'nnz', nnz, 'launch', launch)
+    main_r = time_sweep(cfg, launch, K, W, gw, args.profile_steps, fc_idx)
+    main_r['roofline']['traffic'] = traffic_from_profile(
+        main_r['roofline']['kernel'], args.workload, launch)
     out = {
         'metric': METRIC,
-        'value': value,
+        'value': main_r['value'],
         'unit': 'instance*steps/s',
         'n_gpus': 1,
         'steps': K,
         'warmup': W,
-        'ms_per_step': el / K * 1e3,
+        'ms_per_step': main_r['ms_per_step'],
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': 'f32' if cfg.ring_dtype == 'f32' else
-                 'f32 (%s history)' % cfg.ring_dtype,
+                 'f32 state, %s history' % cfg.ring_dtype,
         'data': 'synthetic',
         'config': {
             'workload': '%s: %s' % (cfg.name, cfg.description),
-            'n_node': N, 'n_inst': P, 'nnz': nnz, 'horizon': cfg.horizon,
-            'dt_ms': cfg.scheme.dt, 'path': args.path, 'launch': launch,
+            'n_node': N, 'n_inst': P, 'nnz': nnz,
+            'delay_bins': cfg.delay_bins, 'dt_ms': cfg.scheme.dt,
+            'launch': launch, 'graph_window': gw,
         },
-        'roofline': {
-            'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
-            'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
-            'traffic': traffic,
-            'kernel': 'tvbx_step',
-            'bytes_per_instance_step': bytes_inst_step,
-            'launch_ms': ms_kern / launches,
-            'launches': launches,
-        },
+        'roofline': main_r['roofline'],
     }
+    plain_r = None
+    if args.plain != 'none' and args.plain != args.workload:
+        pcfg = configs.ALL[args.plain]()
+        pl = launch_of(pcfg, None)
+        log('plain workload', pcfg.name, 'launch', pl)
+        plain_r = time_sweep(pcfg, pl, K, W, gw, args.profile_steps, fc_idx)
+        plain_r['roofline']['traffic'] = traffic_from_profile(
+            plain_r['roofline']['kernel'], args.plain, pl)
+        out['plain'] = {'workload': '%s: %s' % (pcfg.name, pcfg.description),
+                        'value': plain_r['value'],
+                        'ms_per_step': plain_r['ms_per_step'],
+                        'launch': pl, 'roofline': plain_r['roofline']}
+        out['vs_plain'] = main_r['value'] / plain_r['value']
+    if not args.no_grad:
+        out['grad'] = time_gradient()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        n_inst = min(P, threads)
-        v, el_c = cpu_baseline(cfg, x0, n_inst, args.cpu_steps, threads)
-        out['cpu_baseline'] = {
-            'value': v, 'unit': 'instance*steps/s', 'cores': threads,
-            'kind': 'port',
-            'sample': '%d instances x %d steps of %s on %d threads (float32 '
-                      'oracle, %.1f s)' % (n_inst, args.cpu_steps, cfg.name,
-                                           threads, el_c),
-        }
+        pname = args.plain if args.plain != 'none' else args.workload
+        pcfg = configs.ALL[pname]()
+        if pcfg.delay_bins:
+            pcfg = configs.ALL['jansenrit998']()
+        base, fc64 = cpu_legs(pcfg, fc_idx, W + K, min(args.cpu_inst, P),
+                              args.cpu_steps, threads)
+        out['cpu_baseline'] = base
+        if fc64 is not None and main_r['fc'] is not None:
+            err = {'instances': [int(p) for p in fc_idx],
+                   'samples': (W + K) // cfg.tavg_period,
+                   'mixed' if cfg.delay_bins else 'main':
+                       float(np.abs(main_r['fc'] - fc64).max())}
+            if plain_r is not None and plain_r['fc'] is not None:
+                err['plain'] = float(np.abs(plain_r['fc'] - fc64).max())
+            out['fc_err_vs_fp64'] = err
     print(json.dumps(out), flush=True)
 
 
