@@ -262,3 +262,24 @@ def test_configs_build():
     assert c3.n_inst == 1024
     sl = c1.sliced(64)
     assert sl.csr.n_group == 1 and sl.n_inst == 64
+
+
+def test_mpr_workload_starts_in_a_stable_basin():
+    """Every instance of the gradient workload (MPR N=84, Euler-Maruyama at
+    dt 0.1 ms, coupling 0 .. 1) stays finite over its 20000 steps from the
+    model's initial-condition box (float32 oracle on a sample)."""
+    import oracle as O
+    from tvb_hpc import configs
+    cfg = configs.ALL['mpr84']()
+    idx = np.arange(0, cfg.n_inst, 512)
+    csr = cfg.csr
+    ref = O.OracleSim('MontbrioPazoRoxin', 'Linear', csr.nnode, csr.row,
+                      csr.col, csr.weight, csr.delays, len(idx),
+                      csr.max_delay + 2, cfg.scheme.dt, scheme='em',
+                      params={}, cparams={'a': np.asarray(
+                          cfg.params['c:a'])[idx]},
+                      sigma=0.01, dtype=np.float32, n_threads=4)
+    ref.set_state(cfg.initial_state()[..., idx])
+    ref.init_history('zeros')
+    ref.run(20000, seed=5)
+    assert np.isfinite(ref.get_state()).all()
