@@ -118,8 +118,9 @@ typedef struct tvbx_sim_desc {
                         *    the module's emitted order. */
   int ipl;             /* sweep instances per lane (1, 2, 4); must match the
                         * module; group_size must be a multiple of 64 * ipl */
-  int ring_type;       /* ring element: 0 float32, 1 bfloat16, 2 float16;
-                        * must match the module */
+  int ring_type;       /* ring element: 0 float32, 1 bfloat16, 2 float16,
+                        * 3 8-bit fixed point of ring_scale * pre(y) in
+                        * [0, 1]; must match the module */
   float ring_scale;    /* reduced-precision rings store ring_scale * pre(y) */
   int xcd_map;         /* 1: number the step kernels' workgroups so that each
                         * of the 8 XCDs (which take workgroup ids round

@@ -104,8 +104,12 @@ class OracleSpec(ctypes.Structure):
 
 
 def quantize(x, ring_dtype):
-    "round-to-nearest-even to bfloat16 / float16, returned as float32"
+    """round-to-nearest-even to bfloat16 / float16, returned as float32;
+    'u8': the integer round(255 clamp(x, 0, 1))"""
     x = np.asarray(x, np.float32)
+    if ring_dtype == 'u8':
+        c = np.minimum(np.maximum(x, np.float32(0)), np.float32(1))
+        return np.rint(c * np.float32(255)).astype(np.float32)
     if ring_dtype == 'f16':
         return x.astype(np.float16).astype(np.float32)
     if ring_dtype == 'bf16':
@@ -239,7 +243,9 @@ class OracleSim:
         self.order = {'cuda': 0, 'reference': 1}[order]
         # emulation of a reduced-precision history (float32 oracle only)
         self.ring_dtype = ring_dtype
-        self.ring_q = {'f32': 0, 'bf16': 1, 'f16': 2}[ring_dtype]
+        self.ring_q = {'f32': 0, 'bf16': 1, 'f16': 2, 'u8': 3}[ring_dtype]
+        if self.ring_q == 3 and not ring_pre:
+            raise ValueError('an 8-bit history holds ring_scale * pre(y)')
         self.ring_pre = bool(ring_pre)
         self.ring_scale = float(ring_scale)
         self.N, self.P, self.H, self.R = int(N), int(P), int(H), int(R)

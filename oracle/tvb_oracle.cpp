@@ -78,7 +78,8 @@ typedef struct {
                                   //    step n rewrites it
   void *stale;                    // T[P][2][R][N] (order 1): old content of
                                   // the slot being rewritten, by step parity
-  int ring_q;                     // emulate a reduced-precision history:
+  int ring_q;                     // emulate a reduced-precision history
+                                  // (1 bf16, 2 f16, 3 8-bit fixed point):
                                   // 0 none, 1 bfloat16, 2 float16
   int ring_pre;                   // with ring_q: the history holds
                                   // Q(ring_scale * pre(y)) (else Q(y))
@@ -171,9 +172,17 @@ inline float q_f16(float f) {
   return std::nearbyint(f / ulp) * ulp;
 }
 
+// 8-bit fixed point of a [0, 1] value: the integer round(255 clamp(v)),
+// to nearest even (the scale 1/255 is applied to the coupling sum)
+inline float q_u8(float f) {
+  const float c = std::fmin(std::fmax(f, 0.0f), 1.0f);
+  return std::nearbyint(c * 255.0f);
+}
+
 template <typename T> inline T quant(int q, T v) {
   if (q == 1) return T(q_bf16(float(v)));
   if (q == 2) return T(q_f16(float(v)));
+  if (q == 3) return T(q_u8(float(v)));
   return v;
 }
 
@@ -404,7 +413,10 @@ void run_model(OracleSpec *sp) {
   long long written = 0;
   const bool qpre = sp->ring_q && sp->ring_pre;
   const T rscale = T(float(sp->ring_scale));
-  const T runscale = T(1) / rscale;
+  // the device multiplies the sum back by 1 / ring_scale (bf16 / f16) or
+  // 1 / (255 ring_scale) (8-bit), computed in float
+  const T runscale = sp->ring_q == 3 ? T(1.0f / (255.0f * float(rscale)))
+                                     : T(1) / rscale;
 
 #pragma omp parallel for schedule(dynamic, 1) num_threads(sp->n_threads > 0 ? sp->n_threads : omp_get_max_threads()) reduction(max : written)
   for (int p = 0; p < P; ++p) {

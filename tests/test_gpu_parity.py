@@ -91,12 +91,13 @@ def test_trajectory_parity_wide_lanes(model, cfun, dt, cparams, launch):
     assert not sim.flags().any()
 
 
-@pytest.mark.parametrize('ring_dtype', ['bf16', 'f16'])
-@pytest.mark.parametrize('ipl', [1, 4])
+@pytest.mark.parametrize('ring_dtype', ['bf16', 'f16', 'u8'])
+@pytest.mark.parametrize('ipl', [1, 2, 4])
 def test_reduced_precision_ring_parity(ring_dtype, ipl):
-    """bf16 / f16 history holding 200 * sigmoid(y) (1 / cmax scaling) vs the
-    float32 oracle emulating the same rounding.  Rounding to 8 or 11 bits can
-    resolve a last-bit difference of expf either way, so the bar is 1e-4."""
+    """bf16 / f16 / 8-bit fixed-point history holding 200 * sigmoid(y)
+    (1 / cmax scaling, values in [0, 1]) vs the float32 oracle emulating the
+    same rounding.  Rounding can resolve a last-bit difference of expf
+    either way, so the bar is 1e-4."""
     sim, ref, noise = _pair('JansenRit', 'Sigmoidal', 0.05, {'a': 0.3},
                             P=200, steps=600, ring_dtype=ring_dtype,
                             ring_scale=200.0,

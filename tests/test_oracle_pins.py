@@ -386,3 +386,56 @@ def test_hackathon_cuda_order_fails_speed_trend():
     ds, dc = _hackathon('cuda')
     assert ds < 350.0
     assert dc < -500.0
+
+
+def test_u8_history_quantiser():
+    """8-bit fixed point of [0, 1] values: integers 0 .. 255 within half a
+    step of 255 x, monotone, exact on the k / 255 grid, clamped outside."""
+    x = np.linspace(-0.2, 1.2, 20001).astype(np.float32)
+    q = O.quantize(x, 'u8')
+    assert np.all(q == np.round(q)) and q.min() == 0 and q.max() == 255
+    inside = (x >= 0) & (x <= 1)
+    assert np.all(np.abs(q[inside] - 255 * x[inside].astype(np.float64))
+                  <= 0.5 + 1e-4)
+    assert np.all(np.diff(q) >= 0)
+    k = np.arange(256, dtype=np.float32)
+    assert np.array_equal(O.quantize(k / np.float32(255), 'u8'), k)
+    assert np.all(q[x < 0] == 0) and np.all(q[x > 1] == 255)
+
+
+def test_u8_history_error_bound():
+    """One step of a Jansen-Rit network whose history holds the 8-bit fixed
+    point of the scaled sigmoid: every coupling sum is within
+    sum_e |w_e| * 0.5 / 255 / ring_scale of the exact one, so the state
+    after one Euler step moves by at most dt * |df/dc| * a times that."""
+    from tvb_hpc import coupling, model, network
+    W, L = network.synth_connectome(30, 0.3, 5, lmin=1.0, lmax=40.0)
+    csr = network.CSR.from_dense(W, L, [2.0], 0.05)
+    P = 16
+    cf = coupling.Sigmoidal(model.JansenRit())
+    cmax = cf.param['cmax']
+    rng = np.random.default_rng(0)
+    m = model.JansenRit()
+    x0 = m.initial_state(30, P, rng)
+    out = []
+    for rd in ('f32', 'u8'):
+        ref = O.OracleSim('JansenRit', 'Sigmoidal', 30, csr.row, csr.col,
+                          csr.weight, csr.delays, P, csr.max_delay + 2, 0.05,
+                          scheme='euler', cparams={'a': 300.0},
+                          dtype=np.float32, ring_dtype=rd, ring_pre=True,
+                          ring_scale=1.0 / cmax)
+        ref.set_state(x0)
+        ref.init_history('constant')
+        ref.run(1)
+        out.append(ref.get_state())
+    rowsum = np.add.reduceat(np.abs(csr.weight), csr.row[:-1])
+    dc = 300.0 * rowsum * 0.5 / 255 * cmax        # bound on |c_u8 - c|
+    # JR couples into dy4/dt = A a (mu + ... + c): |df4/dc| = A a = 0.325;
+    # plus the float32 spacing of y4 for the final rounding
+    bound = 0.05 * 3.25 * 0.1 * dc[:, None] * (1 + 1e-3) + \
+        2 * np.spacing(np.abs(out[0][4]))
+    d = np.abs(out[1] - out[0])
+    assert np.all(d[4] <= bound), (d[4].max(), bound.max())
+    assert d[4].max() > 10 * np.spacing(np.abs(out[0][4])).max()
+    assert np.all(d[[0, 1, 2, 3, 5]] <= 1e-9)
+    assert d[4].max() > 0

@@ -8,7 +8,8 @@
 //   TVBX_USES_POST   1: pre() reads post_syn
 //   TVBX_ORDER_REF   1: the loopy kernel's read-before-write order
 //   TVBX_IPL         sweep instances per lane (1, 2 or 4)
-//   TVBX_RING_T      ring element: 0 float32, 1 bfloat16, 2 float16
+//   TVBX_RING_T      ring element: 0 float32, 1 bfloat16, 2 float16,
+//                    3 8-bit fixed point (value * 255 of a [0, 1] signal)
 //   TVBX_UNROLL      delayed gathers issued together per CSR row group
 //   TVBX_MIN_WAVES   waves per SIMD the blocked kernel's registers allow
 //   struct tvbx_lane_params + tvbx_load_lane()   per-instance parameters
@@ -34,7 +35,10 @@
 //
 // Precision: state, coupling sums, dfun and the update are float32; the
 // ring may be bfloat16 or float16 (round to nearest even), in which case it
-// stores ring_scale * pre(y) and the sum is multiplied back by ring_unscale.
+// stores ring_scale * pre(y) and the sum is multiplied back by ring_unscale,
+// or 8-bit fixed point: round(255 * clamp(ring_scale * pre(y), 0, 1)) for a
+// pre() bounded by 1 / ring_scale, read back as the integer (exact in
+// float32) with 1 / 255 folded into ring_unscale.
 
 
 #if TVBX_RING_T == 0
@@ -52,10 +56,18 @@ __device__ __forceinline__ tvbx_rt tvbx_f2r(float v) {
   u += 0x7fffu + ((u >> 16) & 1u);
   return (tvbx_rt)(u >> 16);
 }
-#else
+#elif TVBX_RING_T == 2
 typedef _Float16 tvbx_rt;
 __device__ __forceinline__ float tvbx_r2f(tvbx_rt v) { return (float)v; }
 __device__ __forceinline__ tvbx_rt tvbx_f2r(float v) { return (_Float16)v; }
+#else
+typedef unsigned char tvbx_rt;
+__device__ __forceinline__ float tvbx_r2f(tvbx_rt v) { return (float)v; }
+__device__ __forceinline__ tvbx_rt tvbx_f2r(float v) {
+  // clamp (NaN to 0), scale to [0, 255], round to nearest even
+  const float c = fminf(fmaxf(v, 0.0f), 1.0f);
+  return (tvbx_rt)__float2uint_rn(c * 255.0f);
+}
 #endif
 
 // TVBX_IPL consecutive float32 values
@@ -83,37 +95,21 @@ __device__ __forceinline__ void tvbx_stf(float *__restrict__ p,
 #endif
 }
 
-// TVBX_IPL consecutive ring elements as float32
-__device__ __forceinline__ void tvbx_ldr(const tvbx_rt *__restrict__ p,
-                                         float (&o)[TVBX_IPL]) {
-#if TVBX_RING_T == 0
-  tvbx_ldf(p, o);
-#else
-#if TVBX_IPL == 4
-  const uint2 v = *reinterpret_cast<const uint2 *>(p);
-  const unsigned w[2] = {v.x, v.y};
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const unsigned short h = (unsigned short)(w[q >> 1] >> (16 * (q & 1)));
-    o[q] = tvbx_r2f(*reinterpret_cast<const tvbx_rt *>(&h));
-  }
-#elif TVBX_IPL == 2
-  const unsigned v = *reinterpret_cast<const unsigned *>(p);
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const unsigned short h = (unsigned short)(v >> (16 * q));
-    o[q] = tvbx_r2f(*reinterpret_cast<const tvbx_rt *>(&h));
-  }
-#else
-  o[0] = tvbx_r2f(*p);
-#endif
-#endif
-}
-
 __device__ __forceinline__ void tvbx_str(tvbx_rt *__restrict__ p,
                                          const float (&v)[TVBX_IPL]) {
 #if TVBX_RING_T == 0
   tvbx_stf(p, v);
+#elif TVBX_RING_T == 3
+  unsigned u = 0u;
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q) u |= (unsigned)tvbx_f2r(v[q]) << (8 * q);
+#if TVBX_IPL == 4
+  *reinterpret_cast<unsigned *>(p) = u;
+#elif TVBX_IPL == 2
+  *reinterpret_cast<unsigned short *>(p) = (unsigned short)u;
+#else
+  *p = (tvbx_rt)u;
+#endif
 #else
   unsigned short h[TVBX_IPL];
 #pragma unroll
@@ -144,6 +140,14 @@ typedef float2 tvbx_raw;
 #else
 typedef float tvbx_raw;
 #endif
+#elif TVBX_RING_T == 3
+#if TVBX_IPL == 4
+typedef unsigned tvbx_raw;
+#elif TVBX_IPL == 2
+typedef unsigned short tvbx_raw;
+#else
+typedef unsigned char tvbx_raw;
+#endif
 #else
 #if TVBX_IPL == 4
 typedef uint2 tvbx_raw;
@@ -160,7 +164,11 @@ __device__ __forceinline__ tvbx_raw tvbx_ldraw(const tvbx_rt *__restrict__ p) {
 
 __device__ __forceinline__ void tvbx_unpack(const tvbx_raw &v,
                                             float (&o)[TVBX_IPL]) {
-#if TVBX_RING_T == 0
+#if TVBX_RING_T == 3
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q)
+    o[q] = (float)(((unsigned)v >> (8 * q)) & 0xffu);
+#elif TVBX_RING_T == 0
 #if TVBX_IPL == 4
   o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
 #elif TVBX_IPL == 2
@@ -182,6 +190,12 @@ __device__ __forceinline__ void tvbx_unpack(const tvbx_raw &v,
     o[q] = tvbx_r2f(*reinterpret_cast<const tvbx_rt *>(&h));
   }
 #endif
+}
+
+// TVBX_IPL consecutive ring elements as float32
+__device__ __forceinline__ void tvbx_ldr(const tvbx_rt *__restrict__ p,
+                                         float (&o)[TVBX_IPL]) {
+  tvbx_unpack(tvbx_ldraw(p), o);
 }
 
 // normals for state variables [s0, s0 + 4) of (instance p, step n, node i)

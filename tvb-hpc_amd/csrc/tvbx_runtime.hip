@@ -123,7 +123,15 @@ static float f16_to_f32(uint16_t h) {
   return f;
 }
 
-static size_t ring_esz(int ring_type) { return ring_type == 0 ? 4 : 2; }
+static size_t ring_esz(int ring_type) {
+  return ring_type == 0 ? 4 : ring_type == 3 ? 1 : 2;
+}
+
+// 8-bit fixed point of a [0, 1] value: round(255 clamp(v)), to nearest even
+static uint8_t f32_to_u8(float f) {
+  const float c = std::fmin(std::fmax(f, 0.0f), 1.0f);  // NaN -> 0
+  return (uint8_t)std::nearbyint(c * 255.0f);
+}
 
 // ------------------------------------------------------------------ objects
 
@@ -669,7 +677,9 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
     const long long gy = (d.n_node + d.node_block - 1) / d.node_block;
     a.xcd_map = d.xcd_map && (gx * gy) % 8 == 0;
   }
-  a.ring_unscale = d.ring_type == 0 ? 1.0f : 1.0f / d.ring_scale;
+  a.ring_unscale = d.ring_type == 0   ? 1.0f
+                   : d.ring_type == 3 ? 1.0f / (255.0f * d.ring_scale)
+                                      : 1.0f / d.ring_scale;
 }
 
 // reference ordering: the stale block read by step `step`
@@ -709,7 +719,8 @@ extern "C" int tvbx_sim_create(tvbx_ctx *c, tvbx_module *m,
   CHECK(!d.fc || d.tavg_period > 0, "fc needs the monitor (tavg_period > 0)");
   CHECK(d.order == 0 || d.order == 1, "order must be 0 or 1");
   CHECK(d.ipl == 1 || d.ipl == 2 || d.ipl == 4, "ipl must be 1, 2 or 4");
-  CHECK(d.ring_type >= 0 && d.ring_type <= 2, "ring_type must be 0, 1 or 2");
+  CHECK(d.ring_type >= 0 && d.ring_type <= 3,
+        "ring_type must be 0, 1, 2 or 3");
   CHECK(d.xcd_map == 0 || d.xcd_map == 1, "xcd_map must be 0 or 1");
   CHECK(d.ring_type == 0 || (std::isfinite(d.ring_scale) && d.ring_scale > 0),
         "ring_scale must be finite and positive");
@@ -942,6 +953,15 @@ extern "C" int tvbx_sim_get_state(tvbx_sim *s, float *x) {
 static int ring_up(tvbx_sim *s, void *dst, const float *src, size_t rows) {
   if (s->esz == 4) return up2d(s, dst, src, rows);
   const int P = s->d.n_inst;
+  if (s->esz == 1) {
+    std::vector<uint8_t> t8(rows * (size_t)P);
+    for (size_t k = 0; k < t8.size(); ++k) t8[k] = f32_to_u8(src[k]);
+    HIPCHK(hipMemcpy2DAsync(dst, (size_t)s->pitch, t8.data(), (size_t)P,
+                            (size_t)P, rows, hipMemcpyHostToDevice,
+                            s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return TVBX_OK;
+  }
   std::vector<uint16_t> tmp(rows * (size_t)P);
   for (size_t k = 0; k < tmp.size(); ++k)
     tmp[k] = s->d.ring_type == 1 ? f32_to_bf16(src[k]) : f32_to_f16(src[k]);
@@ -955,6 +975,15 @@ static int ring_up(tvbx_sim *s, void *dst, const float *src, size_t rows) {
 static int ring_down(tvbx_sim *s, float *dst, const void *src, size_t rows) {
   if (s->esz == 4) return down2d(s, dst, src, rows);
   const int P = s->d.n_inst;
+  if (s->esz == 1) {
+    std::vector<uint8_t> t8(rows * (size_t)P);
+    HIPCHK(hipMemcpy2DAsync(t8.data(), (size_t)P, src, (size_t)s->pitch,
+                            (size_t)P, rows, hipMemcpyDeviceToHost,
+                            s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    for (size_t k = 0; k < t8.size(); ++k) dst[k] = (float)t8[k] / 255.0f;
+    return TVBX_OK;
+  }
   std::vector<uint16_t> tmp(rows * (size_t)P);
   HIPCHK(hipMemcpy2DAsync(tmp.data(), (size_t)P * 2, src, (size_t)s->pitch * 2,
                           (size_t)P * 2, rows, hipMemcpyDeviceToHost,

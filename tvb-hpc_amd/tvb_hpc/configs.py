@@ -176,8 +176,9 @@ def jansenrit998(n_inst=1024, n_steps=10000, tavg_period=20):
 def jansenrit998_mixed(n_inst=1024, n_steps=10000, tavg_period=20,
                        ring_dtype='bf16', bins=32):
     """Jansen-Rit N=998 with delays quantised to `bins` values and a
-    reduced-precision history ring (bf16 / f16) holding 1/cmax-scaled
-    sigmoid outputs; state, sums and FC stay float32 / float64."""
+    reduced-precision history ring (bf16 / f16, or 8-bit fixed point) holding
+    1/cmax-scaled sigmoid outputs in [0, 1]; state, sums and FC stay
+    float32 / float64."""
     cfg = jansenrit998(n_inst, n_steps, tavg_period)
     csr, _ = cfg.csr.binned(bins)
     cfg.csr = csr.sorted_by_delay()
@@ -216,7 +217,9 @@ ALL = {'kuramoto76': kuramoto76, 'g2do84': g2do84,
        'jansenrit998_bf16': lambda **kw: jansenrit998_mixed(
            ring_dtype='bf16', **kw),
        'jansenrit998_f16': lambda **kw: jansenrit998_mixed(
-           ring_dtype='f16', **kw)}
+           ring_dtype='f16', **kw),
+       'jansenrit998_u8': lambda **kw: jansenrit998_mixed(
+           ring_dtype='u8', **kw)}
 
 
 def all_specs():

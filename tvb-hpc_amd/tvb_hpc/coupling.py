@@ -29,6 +29,11 @@ class BaseCoupling:
     pre_sum = ''
     post_sum = ''
 
+    def pre_bounds(self):
+        """(lo, hi) bounding every pre_sum value, or None when unbounded;
+        an 8-bit fixed-point history needs it."""
+        return None
+
     def __init__(self, model: BaseModel):
         self.model = model
         self.param = dict(type(self).param)
@@ -106,6 +111,13 @@ class Sigmoidal(BaseCoupling):
     param = {'cmin': 0, 'cmax': 0.005, 'midpoint': 6, 'r': 1, 'a': 0.56}
     pre_sum = 'cmax / (1 + exp(r * (midpoint - pre_syn)))',
     post_sum = 'a * sum',
+
+    def pre_bounds(self):
+        "pre_sum lies in [0, cmax] (for cmax >= 0 a literal)"
+        cmax = self.param.get('cmax')
+        if isinstance(cmax, (int, float)) and cmax >= 0:
+            return 0.0, float(cmax)
+        return None
 
 
 class Diff(Linear):

@@ -42,8 +42,9 @@ from .scheme import EulerMaryuyamaStep, EulerStep
 
 CSRC = os.path.normpath(os.path.join(os.path.dirname(__file__), '..', 'csrc'))
 KINDS = ('const', 'glob', 'inst', 'node', 'ni')
-# ring element types: float32, bfloat16, float16
-RING_TYPES = {'f32': 0, 'bf16': 1, 'f16': 2}
+# ring element types: float32, bfloat16, float16, 8-bit fixed point of a
+# [0, 1] value (for couplings whose pre() is bounded)
+RING_TYPES = {'f32': 0, 'bf16': 1, 'f16': 2, 'u8': 3}
 
 # hipRTC / hipcc options shared by the run-time and ahead-of-time builds:
 # no contraction (the only fused multiply-add is the explicit fmaf of the
@@ -135,6 +136,10 @@ class KernelSpec:
                              "ring='coupled'")
         self.ring_pre = bool(ring_pre)
         self.uses_post = uses_post and not self.ring_pre
+        if ring_dtype == 'u8' and not (self.ring_pre and
+                                       cfun.pre_bounds() is not None):
+            raise ValueError('an 8-bit history needs a ring holding a '
+                             'bounded pre(y) (e.g. Sigmoidal coupling)')
         self.kinds = {}
         kinds = dict(kinds or {})
         mstate, minput, mparam = model.names

@@ -293,6 +293,13 @@ class Simulation:
                                     unroll=unroll, min_waves=min_waves)
         if ring_scale != 1.0 and not self.spec.ring_pre:
             raise ValueError('ring_scale needs a ring that holds pre(y)')
+        if ring_dtype == 'u8':
+            lo, hi = cfun.pre_bounds()
+            if lo < 0 or hi * ring_scale > 1.0 + 1e-6:
+                raise ValueError('an 8-bit history holds values in [0, 1]: '
+                                 'ring_scale * pre must stay within it '
+                                 '(pre in [%g, %g], ring_scale %g)'
+                                 % (lo, hi, ring_scale))
         self.values = resolved
         self.ctx = context or Context.default()
         self.module = self.ctx.module(self.spec)
