@@ -23,7 +23,7 @@ grad         gradient evaluations/s of the FC-fit inversion workload (MPR
              N=84, P=4096, 20000 steps, checkpoints every 1000): checkpointed
              forward + adjoint reverse sweep for all instances
 roofline     the step kernel against HBM bandwidth: algorithmic bytes per
-             launch (2 or 4 B per edge gather + per node write, per instance
+             launch (1, 2 or 4 B per edge gather + per node write, per instance
              and step, times the steps of the launch) / mean launch duration
              (HIP events on the simulation's stream); `traffic` is the
              measured HBM bytes per launch from the committed rocprofv3 PMC
@@ -100,7 +100,7 @@ def time_sweep(cfg, launch, K, W, graph_window, profile_steps, fc_idx):
     sim.run(profile_steps)
     sim.sync()
     ms_kern, launches = sim.timing(reset=True)
-    esz = 4 if cfg.ring_dtype == 'f32' else 2
+    esz = {'f32': 4, 'bf16': 2, 'f16': 2, 'u8': 1}[cfg.ring_dtype]
     bytes_inst_step = esz * (nnz + N * sim.spec.C)
     steps_per_launch = profile_steps / max(launches, 1)
     launch_ms = ms_kern / max(launches, 1)

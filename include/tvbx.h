@@ -235,6 +235,54 @@ int tvbx_rng_normals(tvbx_ctx *ctx, unsigned long long seed, float *out,
 int tvbx_corr(tvbx_ctx *ctx, const float *samples, long long n, int N, int P,
               float *corr);
 
+/* ---- batches: subjects with different connectomes in one launch -----------
+ * Reference: the multi-subject harnesses run one compiled kernel per
+ * subject, serially (examples/hcp100.py:75-118, examples/sep645.py:109-153),
+ * and the CUDA sweep gives every work item its own parameters from one
+ * array (dsl/dsl_cuda/example/cuda_run.py:74-80, tmpl8_CUDA.py:58).  Here a
+ * subject is one connectome (node count, CSR, delays, horizon) with its own
+ * instances; every subject's instance tiles and node blocks are work items
+ * of one launch of the batched step kernel, heaviest first.  Instances are
+ * numbered subject after subject (Philox counters, instance parameters,
+ * flags).  One node per wave (node_block == waves <= 8);
+ * steps_per_launch > 1 needs every delay d >= steps_per_launch - 1 and
+ * horizon >= d + steps_per_launch + 1.  No monitor, no injected noise. */
+typedef struct tvbx_batch tvbx_batch;
+typedef struct tvbx_batch_desc {
+  int n_subject;
+  int n_svar, n_ring, n_coupled;
+  int n_pinst, n_pnode, n_pni, n_pglob; /* parameter counts by kind */
+  float dt;
+  int scheme;            /* 0 Euler, 1 Euler-Maruyama */
+  int waves, node_block, steps_per_launch, ipl, ring_type;
+  float ring_scale;
+} tvbx_batch_desc;
+typedef struct tvbx_subject_desc {
+  int n_node, n_inst, nnz, horizon;
+} tvbx_subject_desc;
+int tvbx_batch_create(tvbx_ctx *ctx, tvbx_module *mod,
+                      const tvbx_batch_desc *desc,
+                      const tvbx_subject_desc *subjects, tvbx_batch **out);
+int tvbx_batch_destroy(tvbx_batch *b);
+/* CSR of subject k, as tvbx_sim_set_connectome (one delay row) */
+int tvbx_batch_set_connectome(tvbx_batch *b, int subject, const int32_t *row,
+                              const int32_t *col, const float *weight,
+                              const int32_t *delays);
+/* inst [k][sum P_s]; node [k][sum N_s]; ni per subject [k][N_s][P_s],
+ * subject after subject; glob [k] */
+int tvbx_batch_set_params(tvbx_batch *b, int kind, const float *values,
+                          size_t count);
+int tvbx_batch_set_state(tvbx_batch *b, int subject, const float *x);
+int tvbx_batch_get_state(tvbx_batch *b, int subject, float *x);
+int tvbx_batch_set_step(tvbx_batch *b, long long step);
+int tvbx_batch_get_step(tvbx_batch *b, long long *step);
+int tvbx_batch_set_noise(tvbx_batch *b, int mode, unsigned long long seed);
+int tvbx_batch_init_history(tvbx_batch *b, int mode);
+int tvbx_batch_run(tvbx_batch *b, long long n_steps);
+int tvbx_batch_sync(tvbx_batch *b);
+/* flags [sum P_s]: 1 where an instance's state became non-finite */
+int tvbx_batch_get_flags(tvbx_batch *b, uint32_t *flags);
+
 #ifdef __cplusplus
 }
 #endif

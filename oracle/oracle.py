@@ -100,6 +100,7 @@ class OracleSpec(ctypes.Structure):
         ('order', ctypes.c_int), ('stale', ctypes.c_void_p),
         ('ring_q', ctypes.c_int), ('ring_pre', ctypes.c_int),
         ('ring_scale', ctypes.c_double),
+        ('inst_off', ctypes.c_int),
     ]
 
 
@@ -238,9 +239,12 @@ class OracleSim:
                  dt, scheme='euler', params=None, cparams=None, sigma=None,
                  group_size=None, R=1, dtype=np.float64, tavg_period=0,
                  bold=False, bold_obs=0, n_threads=0, order='cuda',
-                 ring_dtype='f32', ring_pre=False, ring_scale=1.0):
+                 ring_dtype='f32', ring_pre=False, ring_scale=1.0,
+                 inst_off=0):
         self.model, self.cfun = model, cfun
         self.order = {'cuda': 0, 'reference': 1}[order]
+        # Philox counters of instance p use inst_off + p
+        self.inst_off = int(inst_off)
         # emulation of a reduced-precision history (float32 oracle only)
         self.ring_dtype = ring_dtype
         self.ring_q = {'f32': 0, 'bf16': 1, 'f16': 2, 'u8': 3}[ring_dtype]
@@ -407,6 +411,7 @@ class OracleSim:
         sp.ring_q = self.ring_q
         sp.ring_pre = int(self.ring_pre)
         sp.ring_scale = self.ring_scale
+        sp.inst_off = self.inst_off
         f = lib().oracle_run_f64 if self.dtype == np.float64 else \
             lib().oracle_run_f32
         rc = f(ctypes.byref(sp))

@@ -84,6 +84,8 @@ typedef struct {
   int ring_pre;                   // with ring_q: the history holds
                                   // Q(ring_scale * pre(y)) (else Q(y))
   double ring_scale;
+  int inst_off;                   // instance index of the Philox counter is
+                                  // inst_off + p (a subject of a batch)
 } OracleSpec;
 
 }  // extern "C"
@@ -464,7 +466,8 @@ void run_model(OracleSpec *sp) {
             for (int s = 0; s < S; ++s) g[s] = sigma[p];
           for (int s = 0; s < S; ++s) {
             if (sp->noise_mode == 1)
-              xi[s] = philox_normal<T>(sp->seed_lo, sp->seed_hi, n, i, p, s);
+              xi[s] = philox_normal<T>(sp->seed_lo, sp->seed_hi, n, i,
+                                       sp->inst_off + p, s);
             else if (sp->noise_mode == 2)
               xi[s] = noise[(size_t)k * sp->noise_tstride + ((size_t)s * N + i) * P + p];
             else

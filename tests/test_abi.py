@@ -56,6 +56,22 @@ def test_sim_desc_mirror_matches_header():
     assert mirror == names
 
 
+@pytest.mark.parametrize('cname,mirror', [('tvbx_batch_desc', 'BatchDesc'),
+                                           ('tvbx_subject_desc',
+                                            'SubjectDesc')])
+def test_batch_desc_mirrors_match_header(cname, mirror):
+    src = open(HEADER).read()
+    body = src[src.index('typedef struct %s {' % cname):]
+    body = body[:body.index('} %s;' % cname)]
+    fields = re.findall(r'^\s*(int|float)\s+([\w, ]+);', body, re.M)
+    names = []
+    for ty, decl in fields:
+        names += [(ty, n.strip()) for n in decl.split(',')]
+    got = [(('float' if t is ctypes.c_float else 'int'), n)
+           for n, t in getattr(hip, mirror)._fields_]
+    assert got == names
+
+
 def test_args_header_is_plain_c():
     src = open(os.path.join(emit.CSRC, 'tvbx_args.h')).read()
     for bad in ('std::', 'torch', 'template'):

@@ -409,7 +409,7 @@ __device__ __forceinline__ void tvbx_advance(
 #pragma unroll
       for (int s0 = 0; s0 < TVBX_S; s0 += 4) {
         float z[4];
-        tvbx_normals4(a, n, i, p, s0, z);
+        tvbx_normals4(a, n, i, a.inst_off + p, s0, z);
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           if (s0 + r < TVBX_S) xi[s0 + r] = z[r];
@@ -644,18 +644,15 @@ tvbx_step(const tvbx_step_args a) {
 // waves) and keeps its state and tavg accumulator in registers for all the
 // launch's steps.  At most 8 waves per workgroup (the state of 4 instances
 // per lane needs more than 128 registers).
-extern "C" __global__ void __launch_bounds__(512, TVBX_MIN_WAVES)
-tvbx_step_blocked(const tvbx_step_args a) {
+// The blocked launch of one (instance tile bx, node i) pair: n_steps steps
+// of that node with its state in registers.
+__device__ __forceinline__ void tvbx_blocked_run(const tvbx_step_args &a,
+                                                 int bx, int i) {
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  int bx, by;
-  tvbx_block(a, bx, by);
   const int pb = (bx * 64 + lane) * TVBX_IPL;
   const int N = a.n_node;
   const int H = a.horizon;
   const size_t NP = (size_t)N * a.pitch;
-  const int i = __builtin_amdgcn_readfirstlane(by * a.node_block + wave);
-  if (i >= N) return;
   const int group = __builtin_amdgcn_readfirstlane(
       (bx * 64 * TVBX_IPL) / a.group_size);
   const unsigned *__restrict__ cd = a.coldelay + (size_t)group * a.nnz;
@@ -712,6 +709,38 @@ tvbx_step_blocked(const tvbx_step_args a) {
 #pragma unroll
   for (int q = 0; q < TVBX_IPL; ++q)
     if ((bad >> q) & 1u) a.bad[pb + q] = 1u;
+}
+
+extern "C" __global__ void __launch_bounds__(512, TVBX_MIN_WAVES)
+tvbx_step_blocked(const tvbx_step_args a) {
+  int bx, by;
+  tvbx_block(a, bx, by);
+  const int i = __builtin_amdgcn_readfirstlane(by * a.node_block +
+                                               (int)(threadIdx.x >> 6));
+  if (i >= a.n_node) return;
+  tvbx_blocked_run(a, bx, i);
+}
+
+// Batched sweep of subjects with different connectomes (sizes, delays) in
+// one launch: workgroup w takes work item w (subject, instance tile, first
+// node; the host orders the items longest first, and the hardware hands
+// workgroups to free CUs in that order) and runs the blocked launch for
+// its nodes with that subject's arguments.  Every subject's delays allow
+// the launch's n_steps (checked by the host).
+extern "C" __global__ void __launch_bounds__(512, TVBX_MIN_WAVES)
+tvbx_step_batch(const tvbx_batch_args B) {
+  const int w = blockIdx.x;
+  const int subj = __builtin_amdgcn_readfirstlane(B.items[4 * w + 0]);
+  const int tile = __builtin_amdgcn_readfirstlane(B.items[4 * w + 1]);
+  const int node0 = __builtin_amdgcn_readfirstlane(B.items[4 * w + 2]);
+  tvbx_step_args a = B.subj[subj];
+  a.step0 = B.step0;
+  a.n_steps = B.n_steps;
+  a.slot0 = (int)(B.step0 % a.horizon);
+  a.clock = nullptr;
+  const int i = node0 + (int)(threadIdx.x >> 6);
+  if (i >= a.n_node || i >= node0 + a.node_block) return;
+  tvbx_blocked_run(a, tile, i);
 }
 #endif
 

@@ -94,7 +94,26 @@ typedef struct tvbx_step_args {
   /* 1: remap workgroup ids so each XCD takes contiguous instance tiles
    * (grid.x * grid.y is then a multiple of 8) */
   int xcd_map;
+  /* instance index offset and stride of p_inst: 0 and pitch for one
+   * simulation; for a subject of a batch, its first instance in the
+   * batch-wide numbering (Philox counters, p_inst) and the batch's total
+   * pitch */
+  int inst_off;
+  int param_pitch;
 } tvbx_step_args;
+
+/* Argument block of the batched step kernel: per subject (one connectome
+ * shared by its instance tiles) a complete tvbx_step_args with that
+ * subject's sizes and buffer offsets, and a table of work items
+ * (subject, instance tile, first node) in the order the workgroups take
+ * them. */
+typedef struct tvbx_batch_args {
+  const tvbx_step_args *subj;
+  const int *items;  /* [n_items][4]: subject, tile, node0, unused */
+  long long step0;
+  int n_steps;
+  int n_items;
+} tvbx_batch_args;
 
 /* Argument block of the adjoint (reverse) kernel.  `f` carries the forward
  * arguments (sizes, parameters, rec_x / rec_sum of the replayed window,

@@ -148,6 +148,7 @@ struct tvbx_module {
   hipFunction_t hist = nullptr;
   hipFunction_t adj = nullptr;  // present when emitted with the adjoint
   hipFunction_t blocked = nullptr;  // absent in the reference ordering
+  hipFunction_t batch = nullptr;    // absent in the reference ordering
   std::vector<char> image;
 };
 
@@ -306,6 +307,9 @@ static int module_from_image(tvbx_ctx *c, const void *image, size_t size,
   if (hipModuleGetFunction(&m->blocked, m->mod, "tvbx_step_blocked") !=
       hipSuccess)
     m->blocked = nullptr;
+  if (hipModuleGetFunction(&m->batch, m->mod, "tvbx_step_batch") !=
+      hipSuccess)
+    m->batch = nullptr;
   (void)hipGetLastError();
   *out = m;
   return TVBX_OK;
@@ -672,6 +676,8 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
   a.bad = (unsigned *)s->bad.p;
   a.stale = s->stale.p;
   a.ring_scale = d.ring_type == 0 ? 1.0f : d.ring_scale;
+  a.inst_off = 0;
+  a.param_pitch = s->pitch;
   {
     const long long gx = s->pitch / (64 * d.ipl);
     const long long gy = (d.n_node + d.node_block - 1) / d.node_block;
@@ -1747,5 +1753,451 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
                             (size_t)n_gm * N, ok, s->ctx->stream));
   }
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return TVBX_OK;
+}
+
+// ------------------------------------------------------------------ batch
+// Subjects with different connectomes (node count, edges, delays, horizon)
+// and their instance tiles, integrated together: one launch of
+// tvbx_step_batch advances every (subject, instance tile, node block) work
+// item by steps_per_launch steps.
+
+struct tvbx_batch {
+  tvbx_ctx *ctx = nullptr;
+  tvbx_module *mod = nullptr;
+  tvbx_batch_desc d{};
+  std::vector<tvbx_subject_desc> sd;
+  std::vector<int> pitch;                 // per subject
+  std::vector<long long> inst0;           // first instance (padded numbering)
+  std::vector<long long> row_off, edge_off, state_off, node_off, ni_off;
+  std::vector<size_t> ring_off;           // bytes
+  std::vector<char> have_conn;
+  long long P_pitch = 0, P_total = 0, N_total = 0;
+  size_t esz = 4;
+  long long step = 0;
+  int noise_mode = 0;
+  unsigned long long seed = 0;
+  DevBuf row, coldelay, weight, state, ring, p_inst, p_node, p_ni, p_glob, bad,
+      args, items;
+  std::vector<tvbx_step_args> h_args;
+  int n_items = 0;
+  bool args_dirty = true;
+};
+
+static int batch_fill(tvbx_batch *b, DevBuf &buf, float v) {
+  size_t n = buf.bytes / 4;
+  if (!n) return TVBX_OK;
+  hipLaunchKernelGGL(k_fill_f32, dim3(grid1d(n, 256)), dim3(256), 0,
+                     b->ctx->stream, (float *)buf.p, n, v);
+  HIPCHK(hipGetLastError());
+  return TVBX_OK;
+}
+
+// the complete step arguments of subject k
+static void batch_args(tvbx_batch *b, int k, tvbx_step_args &a) {
+  memset(&a, 0, sizeof a);
+  const tvbx_batch_desc &d = b->d;
+  const tvbx_subject_desc &u = b->sd[k];
+  a.n_node = u.n_node;
+  a.n_inst = u.n_inst;
+  a.pitch = b->pitch[k];
+  a.horizon = u.horizon;
+  a.n_ring = d.n_ring;
+  a.nnz = u.nnz;
+  a.group_size = b->pitch[k];
+  a.step0 = b->step;
+  a.slot0 = (int)(b->step % u.horizon);
+  a.node_block = d.node_block;
+  a.dt = d.dt;
+  a.sqrt_dt = sqrtf(d.dt);
+  a.seed_lo = (unsigned)(b->seed & 0xffffffffu);
+  a.seed_hi = (unsigned)(b->seed >> 32);
+  a.noise_mode = b->noise_mode;
+  a.row = (const int *)b->row.p + b->row_off[k];
+  a.coldelay = (const unsigned *)b->coldelay.p + b->edge_off[k];
+  a.weight = (const float *)b->weight.p + b->edge_off[k];
+  a.state = (float *)b->state.p + b->state_off[k];
+  a.ring = (char *)b->ring.p + b->ring_off[k];
+  a.p_inst = (const float *)b->p_inst.p;
+  a.p_node = (const float *)b->p_node.p + b->node_off[k];
+  a.p_ni = (const float *)b->p_ni.p + b->ni_off[k];
+  a.p_glob = (const float *)b->p_glob.p;
+  a.bad = (unsigned *)b->bad.p + b->inst0[k];
+  a.ring_scale = d.ring_type == 0 ? 1.0f : d.ring_scale;
+  a.ring_unscale = d.ring_type == 0   ? 1.0f
+                   : d.ring_type == 3 ? 1.0f / (255.0f * d.ring_scale)
+                                      : 1.0f / d.ring_scale;
+  a.inst_off = (int)b->inst0[k];
+  a.param_pitch = (int)b->P_pitch;
+}
+
+static int batch_upload_args(tvbx_batch *b) {
+  if (!b->args_dirty) return TVBX_OK;
+  for (size_t k = 0; k < b->sd.size(); ++k) batch_args(b, (int)k, b->h_args[k]);
+  HIPCHK(hipMemcpyAsync(b->args.p, b->h_args.data(),
+                        b->h_args.size() * sizeof(tvbx_step_args),
+                        hipMemcpyHostToDevice, b->ctx->stream));
+  HIPCHK(hipStreamSynchronize(b->ctx->stream));
+  b->args_dirty = false;
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_batch_create(tvbx_ctx *c, tvbx_module *m,
+                                 const tvbx_batch_desc *desc,
+                                 const tvbx_subject_desc *subjects,
+                                 tvbx_batch **out) {
+  CHECK(c && m && desc && subjects && out, "null argument");
+  const tvbx_batch_desc &d = *desc;
+  CHECK(m->batch, "module lacks tvbx_step_batch (reference ordering?)");
+  CHECK(d.n_subject >= 1, "need at least one subject");
+  CHECK(d.n_svar > 0 && d.n_ring > 0 && d.n_coupled > 0 &&
+            d.n_coupled <= d.n_ring,
+        "bad variable counts");
+  CHECK(std::isfinite(d.dt) && d.dt > 0, "dt must be finite and positive");
+  CHECK(d.scheme == 0 || d.scheme == 1, "scheme must be 0 or 1");
+  CHECK(d.waves >= 1 && d.waves <= 8 && d.node_block == d.waves,
+        "batches take one node per wave: node_block == waves in [1, 8]");
+  CHECK(d.steps_per_launch >= 1, "steps_per_launch must be positive");
+  CHECK(d.ipl == 1 || d.ipl == 2 || d.ipl == 4, "ipl must be 1, 2 or 4");
+  CHECK(d.ring_type >= 0 && d.ring_type <= 3, "ring_type must be 0 .. 3");
+  CHECK(d.ring_type == 0 || (std::isfinite(d.ring_scale) && d.ring_scale > 0),
+        "ring_scale must be finite and positive");
+  tvbx_batch *b = new tvbx_batch;
+  b->ctx = c;
+  b->mod = m;
+  b->d = d;
+  b->esz = ring_esz(d.ring_type);
+  b->sd.assign(subjects, subjects + d.n_subject);
+  const int tile = 64 * d.ipl;
+  long long rows = 0, edges = 0, st = 0, nodes = 0, ni = 0;
+  size_t rg = 0;
+  for (int k = 0; k < d.n_subject; ++k) {
+    const tvbx_subject_desc &u = b->sd[k];
+    if (!(u.n_node >= 1 && u.n_node <= 65536 && u.n_inst >= 1 &&
+          u.nnz >= 0 && u.horizon >= 2 && u.horizon <= 65537)) {
+      delete b;
+      return fail(TVBX_E_ARG, "subject %d: bad sizes", k);
+    }
+    const int pt = (u.n_inst + tile - 1) / tile * tile;
+    b->pitch.push_back(pt);
+    b->inst0.push_back(b->P_pitch);
+    b->row_off.push_back(rows);
+    b->edge_off.push_back(edges);
+    b->state_off.push_back(st);
+    b->node_off.push_back(nodes);
+    b->ni_off.push_back(ni);
+    b->ring_off.push_back(rg);
+    b->P_pitch += pt;
+    b->P_total += u.n_inst;
+    b->N_total += u.n_node;
+    rows += u.n_node + 1;
+    edges += u.nnz;
+    st += (long long)d.n_svar * u.n_node * pt;
+    nodes += (long long)std::max(d.n_pnode, 1) * u.n_node;
+    ni += (long long)std::max(d.n_pni, 1) * u.n_node * pt;
+    // 256-byte aligned ring blocks
+    rg += ((size_t)u.horizon * d.n_ring * u.n_node * pt * b->esz + 255) / 256 *
+          256;
+  }
+  b->have_conn.assign(d.n_subject, 0);
+  b->h_args.resize(d.n_subject);
+  int rc = TVBX_OK;
+  HIPCHK(hipSetDevice(c->device));
+#define BALLOC(buf, bytes) \
+  if (rc == TVBX_OK) rc = dalloc(b->buf, (size_t)(bytes));
+  BALLOC(row, rows * 4);
+  BALLOC(coldelay, std::max<long long>(edges, 1) * 4);
+  BALLOC(weight, std::max<long long>(edges, 1) * 4);
+  BALLOC(state, st * 4);
+  BALLOC(ring, rg);
+  BALLOC(p_inst, (size_t)std::max(d.n_pinst, 1) * b->P_pitch * 4);
+  BALLOC(p_node, nodes * 4);
+  BALLOC(p_ni, ni * 4);
+  BALLOC(p_glob, (size_t)std::max(d.n_pglob, 1) * 4);
+  BALLOC(bad, b->P_pitch * 4);
+  BALLOC(args, (size_t)d.n_subject * sizeof(tvbx_step_args));
+#undef BALLOC
+  DevBuf *z[] = {&b->state, &b->ring, &b->p_inst, &b->p_node, &b->p_ni,
+                 &b->p_glob, &b->bad};
+  for (DevBuf *x : z)
+    if (rc == TVBX_OK) rc = batch_fill(b, *x, 0.0f);
+  if (rc == TVBX_OK) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = fail(TVBX_E_HIP, "init: %s", hipGetErrorString(e));
+  }
+  if (rc != TVBX_OK) {
+    tvbx_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_batch_destroy(tvbx_batch *b) {
+  if (!b) return TVBX_OK;
+  hipSetDevice(b->ctx->device);
+  hipStreamSynchronize(b->ctx->stream);
+  DevBuf *all[] = {&b->row,    &b->coldelay, &b->weight, &b->state,
+                   &b->ring,   &b->p_inst,   &b->p_node, &b->p_ni,
+                   &b->p_glob, &b->bad,      &b->args,   &b->items};
+  for (DevBuf *x : all) dfree(*x);
+  delete b;
+  return TVBX_OK;
+}
+
+// the work items, heaviest first: subject, instance tile, first node
+static int batch_items(tvbx_batch *b) {
+  const int nb = b->d.node_block, tile = 64 * b->d.ipl;
+  struct It {
+    long long w;
+    int s, x, y;
+  };
+  std::vector<It> its;
+  for (int k = 0; k < b->d.n_subject; ++k) {
+    const tvbx_subject_desc &u = b->sd[k];
+    const int *row = nullptr;
+    std::vector<int> h(u.n_node + 1);
+    HIPCHK(hipMemcpy(h.data(), (const int *)b->row.p + b->row_off[k],
+                     h.size() * 4, hipMemcpyDeviceToHost));
+    row = h.data();
+    for (int y = 0; y < u.n_node; y += nb) {
+      const int y1 = std::min(u.n_node, y + nb);
+      // gathers dominate; the dfun of each node adds a little
+      const long long w = (long long)(row[y1] - row[y]) + 4 * (y1 - y);
+      for (int x = 0; x < b->pitch[k] / tile; ++x) its.push_back({w, k, x, y});
+    }
+  }
+  std::stable_sort(its.begin(), its.end(),
+                   [](const It &a, const It &c) { return a.w > c.w; });
+  std::vector<int> t(its.size() * 4);
+  for (size_t n = 0; n < its.size(); ++n) {
+    t[4 * n] = its[n].s;
+    t[4 * n + 1] = its[n].x;
+    t[4 * n + 2] = its[n].y;
+    t[4 * n + 3] = 0;
+  }
+  int rc = dalloc(b->items, std::max<size_t>(t.size(), 4) * 4);
+  if (rc != TVBX_OK) return rc;
+  HIPCHK(hipMemcpy(b->items.p, t.data(), t.size() * 4, hipMemcpyHostToDevice));
+  b->n_items = (int)its.size();
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_batch_set_connectome(tvbx_batch *b, int k,
+                                         const int32_t *row,
+                                         const int32_t *col,
+                                         const float *weight,
+                                         const int32_t *delays) {
+  CHECK(b && row && k >= 0 && k < b->d.n_subject, "bad argument");
+  const tvbx_subject_desc &u = b->sd[k];
+  const int L = b->d.steps_per_launch;
+  CHECK(u.nnz == 0 || (col && weight && delays), "null argument");
+  CHECK(row[0] == 0 && row[u.n_node] == u.nnz,
+        "subject %d: row[0] must be 0 and row[N] = nnz (%d)", k, u.nnz);
+  for (int i = 0; i < u.n_node; ++i)
+    CHECK(row[i + 1] >= row[i], "subject %d: row must be non-decreasing", k);
+  std::vector<unsigned> cd(std::max(u.nnz, 1), 0u);
+  for (int e = 0; e < u.nnz; ++e) {
+    CHECK(col[e] >= 0 && col[e] < u.n_node, "subject %d: col[%d] = %d", k, e,
+          col[e]);
+    const int dl = delays[e];
+    CHECK(dl >= 0 && dl < 65536 && dl + L + 1 <= u.horizon,
+          "subject %d: delay %d of edge %d needs horizon >= %d (have %d)", k,
+          dl, e, dl + L + 1, u.horizon);
+    CHECK(L == 1 || (dl >= 1 && dl + 1 >= L),
+          "subject %d: delay %d does not allow %d steps per launch", k, dl, L);
+    CHECK(std::isfinite(weight[e]), "subject %d: weight[%d] not finite", k, e);
+    cd[e] = ((unsigned)dl << 16) | (unsigned)col[e];
+  }
+  HIPCHK(hipSetDevice(b->ctx->device));
+  HIPCHK(hipMemcpy((int *)b->row.p + b->row_off[k], row,
+                   (size_t)(u.n_node + 1) * 4, hipMemcpyHostToDevice));
+  if (u.nnz) {
+    HIPCHK(hipMemcpy((unsigned *)b->coldelay.p + b->edge_off[k], cd.data(),
+                     (size_t)u.nnz * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((float *)b->weight.p + b->edge_off[k], weight,
+                     (size_t)u.nnz * 4, hipMemcpyHostToDevice));
+  }
+  b->have_conn[k] = 1;
+  b->n_items = 0;  // rebuilt at the next run
+  return TVBX_OK;
+}
+
+// values: inst [k][P_total] (the subjects' instances one after the other),
+// node [k][sum N] (subject blocks), ni [k][N_s][P_s] per subject block,
+// one after the other, glob [k]
+extern "C" int tvbx_batch_set_params(tvbx_batch *b, int kind,
+                                     const float *v, size_t count) {
+  CHECK(b, "null batch");
+  const tvbx_batch_desc &d = b->d;
+  CHECK(count == 0 || v, "null values");
+  for (size_t n = 0; n < count; ++n)
+    CHECK(std::isfinite(v[n]), "parameter value %zu is not finite", n);
+  HIPCHK(hipSetDevice(b->ctx->device));
+  std::vector<float> h;
+  switch (kind) {
+    case TVBX_P_GLOB:
+      CHECK(count == (size_t)d.n_pglob, "expected %d global values", d.n_pglob);
+      if (count)
+        HIPCHK(hipMemcpy(b->p_glob.p, v, count * 4, hipMemcpyHostToDevice));
+      return TVBX_OK;
+    case TVBX_P_INST: {
+      CHECK(count == (size_t)d.n_pinst * b->P_total,
+            "expected %d x %lld instance values", d.n_pinst, b->P_total);
+      h.assign((size_t)std::max(d.n_pinst, 1) * b->P_pitch, 0.0f);
+      for (int q = 0; q < d.n_pinst; ++q) {
+        long long src = (long long)q * b->P_total;
+        for (int k = 0; k < d.n_subject; ++k)
+          for (int j = 0; j < b->sd[k].n_inst; ++j)
+            h[(size_t)q * b->P_pitch + b->inst0[k] + j] = v[src++];
+      }
+      HIPCHK(hipMemcpy(b->p_inst.p, h.data(), h.size() * 4,
+                       hipMemcpyHostToDevice));
+      return TVBX_OK;
+    }
+    case TVBX_P_NODE: {
+      CHECK(count == (size_t)d.n_pnode * b->N_total,
+            "expected %d x %lld node values", d.n_pnode, b->N_total);
+      h.assign(b->p_node.bytes / 4, 0.0f);
+      size_t src = 0;
+      for (int q = 0; q < d.n_pnode; ++q)
+        for (int k = 0; k < d.n_subject; ++k)
+          for (int i = 0; i < b->sd[k].n_node; ++i)
+            h[b->node_off[k] + (size_t)q * b->sd[k].n_node + i] = v[src++];
+      HIPCHK(hipMemcpy(b->p_node.p, h.data(), h.size() * 4,
+                       hipMemcpyHostToDevice));
+      return TVBX_OK;
+    }
+    case TVBX_P_NI: {
+      size_t need = 0;
+      for (int k = 0; k < d.n_subject; ++k)
+        need += (size_t)d.n_pni * b->sd[k].n_node * b->sd[k].n_inst;
+      CHECK(count == need, "expected %zu node-and-instance values", need);
+      h.assign(b->p_ni.bytes / 4, 0.0f);
+      size_t src = 0;
+      for (int k = 0; k < d.n_subject; ++k)
+        for (int q = 0; q < d.n_pni; ++q)
+          for (int i = 0; i < b->sd[k].n_node; ++i)
+            for (int j = 0; j < b->sd[k].n_inst; ++j)
+              h[b->ni_off[k] + ((size_t)q * b->sd[k].n_node + i) * b->pitch[k] +
+                j] = v[src++];
+      HIPCHK(hipMemcpy(b->p_ni.p, h.data(), h.size() * 4,
+                       hipMemcpyHostToDevice));
+      return TVBX_OK;
+    }
+  }
+  return fail(TVBX_E_ARG, "unknown parameter kind %d", kind);
+}
+
+// x [S][N_s][P_s] of subject k
+extern "C" int tvbx_batch_set_state(tvbx_batch *b, int k, const float *x) {
+  CHECK(b && x && k >= 0 && k < b->d.n_subject, "bad argument");
+  const tvbx_subject_desc &u = b->sd[k];
+  HIPCHK(hipSetDevice(b->ctx->device));
+  HIPCHK(hipMemcpy2D((float *)b->state.p + b->state_off[k],
+                     (size_t)b->pitch[k] * 4, x, (size_t)u.n_inst * 4,
+                     (size_t)u.n_inst * 4, (size_t)b->d.n_svar * u.n_node,
+                     hipMemcpyHostToDevice));
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_batch_get_state(tvbx_batch *b, int k, float *x) {
+  CHECK(b && x && k >= 0 && k < b->d.n_subject, "bad argument");
+  const tvbx_subject_desc &u = b->sd[k];
+  HIPCHK(hipSetDevice(b->ctx->device));
+  HIPCHK(hipStreamSynchronize(b->ctx->stream));
+  HIPCHK(hipMemcpy2D(x, (size_t)u.n_inst * 4,
+                     (const float *)b->state.p + b->state_off[k],
+                     (size_t)b->pitch[k] * 4, (size_t)u.n_inst * 4,
+                     (size_t)b->d.n_svar * u.n_node, hipMemcpyDeviceToHost));
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_batch_set_step(tvbx_batch *b, long long step) {
+  CHECK(b && step >= 0, "bad argument");
+  b->step = step;
+  b->args_dirty = true;
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_batch_get_step(tvbx_batch *b, long long *step) {
+  CHECK(b && step, "bad argument");
+  *step = b->step;
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_batch_set_noise(tvbx_batch *b, int mode,
+                                    unsigned long long seed) {
+  CHECK(b && (mode == 0 || mode == 1), "bad argument");
+  b->noise_mode = mode;
+  b->seed = seed;
+  b->args_dirty = true;
+  return TVBX_OK;
+}
+
+// every subject's ring from its current state (modes as tvbx_sim_init_history)
+extern "C" int tvbx_batch_init_history(tvbx_batch *b, int mode) {
+  CHECK(b && mode >= 0 && mode <= 2, "bad argument");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  b->args_dirty = true;
+  int rc = batch_upload_args(b);
+  if (rc != TVBX_OK) return rc;
+  for (int k = 0; k < b->d.n_subject; ++k) {
+    tvbx_step_args a = b->h_args[k];
+    void *params[] = {&a, &mode};
+    HIPCHK(hipModuleLaunchKernel(b->mod->hist, b->pitch[k] / 64,
+                                 b->sd[k].n_node, 1, 64, 1, 1, 0,
+                                 b->ctx->stream, params, nullptr));
+  }
+  HIPCHK(hipStreamSynchronize(b->ctx->stream));
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_batch_run(tvbx_batch *b, long long n_steps) {
+  CHECK(b && n_steps >= 0, "bad argument");
+  for (int k = 0; k < b->d.n_subject; ++k)
+    CHECK(b->have_conn[k], "subject %d has no connectome", k);
+  HIPCHK(hipSetDevice(b->ctx->device));
+  int rc = TVBX_OK;
+  if (b->n_items == 0) rc = batch_items(b);
+  if (rc == TVBX_OK) rc = batch_upload_args(b);
+  if (rc != TVBX_OK) return rc;
+  const int L = b->d.steps_per_launch;
+  for (long long done = 0; done < n_steps;) {
+    tvbx_batch_args B;
+    B.subj = (const tvbx_step_args *)b->args.p;
+    B.items = (const int *)b->items.p;
+    B.step0 = b->step;
+    B.n_steps = (int)std::min<long long>(L, n_steps - done);
+    B.n_items = b->n_items;
+    void *params[] = {&B};
+    HIPCHK(hipModuleLaunchKernel(b->mod->batch, b->n_items, 1, 1,
+                                 64u * b->d.waves, 1, 1, 0, b->ctx->stream,
+                                 params, nullptr));
+    b->step += B.n_steps;
+    done += B.n_steps;
+  }
+  // the per-subject step0 / slot0 in the table are stale now; the batch
+  // kernel takes them from B, but init_history uses them
+  b->args_dirty = true;
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_batch_sync(tvbx_batch *b) {
+  CHECK(b, "null batch");
+  HIPCHK(hipStreamSynchronize(b->ctx->stream));
+  return TVBX_OK;
+}
+
+// flags [P_total]: 1 where an instance's state became non-finite
+extern "C" int tvbx_batch_get_flags(tvbx_batch *b, uint32_t *flags) {
+  CHECK(b && flags, "bad argument");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  HIPCHK(hipStreamSynchronize(b->ctx->stream));
+  std::vector<uint32_t> h(b->P_pitch);
+  HIPCHK(hipMemcpy(h.data(), b->bad.p, h.size() * 4, hipMemcpyDeviceToHost));
+  size_t o = 0;
+  for (int k = 0; k < b->d.n_subject; ++k)
+    for (int j = 0; j < b->sd[k].n_inst; ++j) flags[o++] = h[b->inst0[k] + j];
   return TVBX_OK;
 }

@@ -97,6 +97,9 @@ def default_launch(cfg):
 def _prefix(cfg):
     # the workload (name and sizes), not the kernel source: a launch shape
     # stays a good starting point across source revisions
+    if hasattr(cfg, 'subjects'):
+        return '%s|S%d|P%d|nnz%d|' % (cfg.name, len(cfg.subjects), cfg.n_inst,
+                                      sum(c.nnz for c, _ in cfg.subjects))
     return '%s|N%d|P%d|nnz%d|' % (cfg.name, cfg.csr.nnode, cfg.n_inst,
                                   cfg.csr.nnz)
 
@@ -171,6 +174,71 @@ def tune(cfg, steps=100, warmup=10, ctx=None, log=None,
             results.append((ms, cand))
             if log:
                 log('%s %.4f ms/step' % (cand, ms))
+    results.sort(key=lambda r: r[0])
+    best = results[0][1]
+    table = _load()
+    table[_key(cfg, ctx.info())] = {'launch': best, 'ms_per_step':
+                                    results[0][0], 'tried': len(results)}
+    os.makedirs(os.path.dirname(TABLE), exist_ok=True)
+    tmp = TABLE + '.%d.tmp' % os.getpid()
+    with open(tmp, 'w') as fd:
+        json.dump(table, fd, indent=1, sort_keys=True)
+    os.replace(tmp, TABLE)
+    return best, results
+
+
+# -- batches of subjects -------------------------------------------------------
+
+def batch_candidates(cfg):
+    "Launch shapes of a subject batch: one node per wave, temporal blocks."
+    dmin = min(int(np.min(c.delays)) for c, _ in cfg.subjects if c.nnz)
+    out = []
+    for spl in sorted({1} | {min(dmin + 1, b) for b in (8, 16, 32)}):
+        for waves in (1, 2, 4, 8):
+            for mw in (1, 4):
+                out.append(dict(ipl=1, waves=waves, steps_per_launch=spl,
+                                min_waves=mw))
+    return out
+
+
+def default_batch_launch(cfg):
+    dmin = min(int(np.min(c.delays)) for c, _ in cfg.subjects if c.nnz)
+    return dict(ipl=1, waves=4, steps_per_launch=min(dmin + 1, 16))
+
+
+def best_batch_launch(cfg, ctx=None):
+    table = _load()
+    if table:
+        from .hip import Context
+        ctx = ctx or Context.default()
+        hit = table.get(_key(cfg, ctx.info()))
+        if hit:
+            return hit['launch']
+    return default_batch_launch(cfg)
+
+
+def tune_batch(cfg, steps=200, warmup=20, ctx=None, log=None):
+    """Time the batch launch shapes; record and return the best."""
+    from .hip import Context
+    ctx = ctx or Context.default()
+    x0 = cfg.initial_states()
+    results = []
+    for cand in batch_candidates(cfg):
+        b = cfg.batch(context=ctx, **cand)
+        for k, x in enumerate(x0):
+            b.set_state(k, x)
+        b.init_history('zeros')
+        b.set_noise(seed=1)
+        b.run(warmup)
+        b.sync()
+        t = time.perf_counter()
+        b.run(steps)
+        b.sync()
+        ms = (time.perf_counter() - t) / steps * 1e3
+        del b
+        results.append((ms, cand))
+        if log:
+            log('%s %.4f ms/step' % (cand, ms))
     results.sort(key=lambda r: r[0])
     best = results[0][1]
     table = _load()

@@ -113,6 +113,74 @@ class SweepConfig:
                           sigma_kind=sk, **kw)
 
 
+@dataclass
+class BatchConfig:
+    """A sweep over subjects with different connectomes, integrated in one
+    launch (``tvb_hpc.hip.Batch``): subjects = [(csr, n_inst), ...]."""
+    name: str
+    model: object
+    cfun: object
+    scheme: object
+    subjects: list
+    n_steps: int
+    params: Dict[str, object] = field(default_factory=dict)
+    kinds: Dict[str, str] = field(default_factory=dict)
+    seed: int = 0
+    description: str = ''
+
+    @property
+    def n_inst(self):
+        return sum(p for _, p in self.subjects)
+
+    def initial_states(self, dtype=np.float32):
+        rng = np.random.default_rng(1000 + self.seed)
+        return [self.model.initial_state(csr.nnode, p, rng, dtype)
+                for csr, p in self.subjects]
+
+    def batch(self, **kw):
+        from .hip import Batch
+        return Batch(self.model, self.cfun, self.scheme, self.subjects,
+                     params=self.params, kinds=self.kinds, **kw)
+
+    def spec(self, **kw):
+        "Kernel spec of the batch (for ahead-of-time builds)."
+        from .emit import KernelSpec
+        kinds = dict(self.kinds)
+        sk = kinds.pop('sigma', None)
+        return KernelSpec(self.model, self.cfun, self.scheme, kinds=kinds,
+                          sigma_kind=sk, **kw)
+
+
+def rww_uneven(n_subjects=32, inst_per_subject=64, n_steps=20000,
+               sizes=(68, 84, 148, 246, 400), seed=2):
+    """Batched sweep, uneven sizes in one launch: reduced Wong-Wang on
+    n_subjects connectomes with N drawn uniformly from `sizes`, each built
+    like the Kuramoto N=76 one (30 % density, lognormal weights, lengths
+    U(10, 200) mm) with its own seed and a conduction speed U(1, 10) m/s;
+    each subject has inst_per_subject instances (one wavefront tile) with
+    G, J_NMDA in [0.1, 0.3] and sigma in [1e-3, 1e-2] drawn uniformly."""
+    rng = np.random.default_rng(seed)
+    dt = 0.1
+    subjects = []
+    for k in range(n_subjects):
+        N = int(rng.choice(sizes))
+        v = float(rng.uniform(1.0, 10.0))
+        W, L = synth_connectome(N, 0.30, seed=100 * seed + k)
+        subjects.append((csr_from(W, L, [v], dt), inst_per_subject))
+    P = n_subjects * inst_per_subject
+    G = rng.uniform(0.0, 0.5, P)
+    J = rng.uniform(0.1, 0.3, P)
+    sigma = rng.uniform(1e-3, 1e-2, P)
+    osc = M.ReducedWongWang()
+    return BatchConfig(
+        'rww_uneven', osc, CF.Linear(osc), SC.EulerMaryuyamaStep(dt),
+        subjects, n_steps,
+        params={'c:a': G, 'm:J': J, 'sigma': sigma},
+        kinds={'c:a': 'inst', 'm:J': 'inst', 'sigma': 'inst'}, seed=seed,
+        description='RWW, %d subjects (N in %s), P=%d, 20000 steps'
+        % (n_subjects, '/'.join(str(s) for s in sizes), P))
+
+
 def csr_from(weights, lengths, speeds, dt, policy='round'):
     return CSR.from_dense(weights, lengths, speeds, dt, policy)
 
@@ -221,6 +289,9 @@ ALL = {'kuramoto76': kuramoto76, 'g2do84': g2do84,
        'jansenrit998_u8': lambda **kw: jansenrit998_mixed(
            ring_dtype='u8', **kw)}
 
+# sweeps over subjects with different connectomes (tvb_hpc.hip.Batch)
+BATCHES = {'rww_uneven': rww_uneven}
+
 
 def all_specs():
     """Kernel specs the builtin configurations use, for every launch width
@@ -231,6 +302,15 @@ def all_specs():
         cfg = fn()
         launches = [dict(ipl=ipl) for ipl in valid_ipls(cfg)]
         launches += table_launches(cfg)
+        for la in launches:
+            sp = cfg.spec(ipl=la['ipl'], unroll=la.get('unroll', 8),
+                          min_waves=la.get('min_waves', 1))
+            if sp.key() not in keys:
+                keys.add(sp.key())
+                specs.append(sp)
+    for fn in BATCHES.values():
+        cfg = fn()
+        launches = [dict(ipl=1)] + table_launches(cfg)
         for la in launches:
             sp = cfg.spec(ipl=la['ipl'], unroll=la.get('unroll', 8),
                           min_waves=la.get('min_waves', 1))

@@ -274,7 +274,8 @@ class KernelSpec:
         lines.append('__device__ __forceinline__ void tvbx_load_lane('
                      'tvbx_lane_params &lp, const tvbx_step_args &a, int p) {')
         for key, kind, idx in fields:
-            src = ('a.p_inst[(size_t)%d * a.pitch + p]' % idx if kind == 'inst'
+            src = ('a.p_inst[(size_t)%d * a.param_pitch + a.inst_off + p]'
+                   % idx if kind == 'inst'
                    else 'a.p_glob[%d]' % idx)
             lines.append('  lp.%s = %s;' % (self.cname(key), src))
         lines.append('}')

@@ -51,6 +51,23 @@ class SimDesc(ctypes.Structure):
                                              ('xcd_map', ctypes.c_int)]
 
 
+class BatchDesc(ctypes.Structure):
+    "Mirror of tvbx_batch_desc (include/tvbx.h)."
+    _fields_ = [(n, ctypes.c_int) for n in (
+        'n_subject', 'n_svar', 'n_ring', 'n_coupled', 'n_pinst', 'n_pnode',
+        'n_pni', 'n_pglob')] + [('dt', ctypes.c_float)] + [
+        (n, ctypes.c_int) for n in ('scheme', 'waves', 'node_block',
+                                    'steps_per_launch', 'ipl',
+                                    'ring_type')] + [
+        ('ring_scale', ctypes.c_float)]
+
+
+class SubjectDesc(ctypes.Structure):
+    "Mirror of tvbx_subject_desc (include/tvbx.h)."
+    _fields_ = [(n, ctypes.c_int) for n in ('n_node', 'n_inst', 'nnz',
+                                            'horizon')]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _LL = ctypes.c_longlong
@@ -92,6 +109,20 @@ _SIGS = {
     'tvbx_sim_timing': [_P, _P, _P, _I],
     'tvbx_rng_normals': [_P, ctypes.c_ulonglong, _P, _SZ],
     'tvbx_corr': [_P, _P, _LL, _I, _I, _P],
+    'tvbx_batch_create': [_P, _P, ctypes.POINTER(BatchDesc),
+                          ctypes.POINTER(SubjectDesc), _P],
+    'tvbx_batch_destroy': [_P],
+    'tvbx_batch_set_connectome': [_P, _I, _P, _P, _P, _P],
+    'tvbx_batch_set_params': [_P, _I, _P, _SZ],
+    'tvbx_batch_set_state': [_P, _I, _P],
+    'tvbx_batch_get_state': [_P, _I, _P],
+    'tvbx_batch_set_step': [_P, _LL],
+    'tvbx_batch_get_step': [_P, _P],
+    'tvbx_batch_set_noise': [_P, _I, ctypes.c_ulonglong],
+    'tvbx_batch_init_history': [_P, _I],
+    'tvbx_batch_run': [_P, _LL],
+    'tvbx_batch_sync': [_P],
+    'tvbx_batch_get_flags': [_P, _P],
 }
 
 _lib = None
@@ -562,6 +593,178 @@ class Simulation:
 
 # ---------------------------------------------------------------------------
 # reference-compatible executor
+
+class Batch:
+    """Subjects with different connectomes, integrated in one launch.
+
+    subjects: [(csr, n_inst), ...] -- each ``network.CSR`` with one delay
+        row; subject k's instances are numbered after those of subjects
+        0 .. k-1 (P_total instances in all).
+    params: {name: value}; instance parameters as (P_total,) arrays, node
+        parameters as a list of (N_k,) arrays, scalars for globals.
+    launch: ipl, waves (= nodes per workgroup), steps_per_launch, min_waves
+        (see tvb_hpc.autotune).
+    Philox noise only (``set_noise``); no monitor.
+    """
+
+    def __init__(self, model, cfun, scheme, subjects, params=None,
+                 kinds=None, ring_dtype='f32', ring_scale=1.0, waves=1,
+                 steps_per_launch=1, ipl=1, min_waves=1, unroll=8, dt=None,
+                 context=None):
+        self.model, self.cfun, self.scheme = model, cfun, scheme
+        self.subjects = [(csr, int(p)) for csr, p in subjects]
+        self.P_total = sum(p for _, p in self.subjects)
+        params = dict(params or {})
+        kinds = dict(kinds or {})
+        probe = emit.KernelSpec(model, cfun, scheme, sigma_kind=None)
+        resolved = {}
+        for name, val in params.items():
+            key = _resolve(probe, name) if name != 'sigma' else 'sigma'
+            resolved[key] = val
+            if key not in kinds:
+                a = np.asarray(val)
+                kinds[key] = 'inst' if a.shape == (self.P_total, ) else \
+                    'const' if a.ndim == 0 else 'node'
+        sigma_kind = kinds.pop('sigma', None)
+        if sigma_kind is None and getattr(scheme, 'sigma', None) is not None:
+            sigma_kind = 'const'
+        self.spec = emit.KernelSpec(model, cfun, scheme, kinds=kinds,
+                                    sigma_kind=sigma_kind, ipl=ipl,
+                                    ring_dtype=ring_dtype, unroll=unroll,
+                                    min_waves=min_waves)
+        if ring_dtype == 'u8':
+            lo, hi = cfun.pre_bounds()
+            if lo < 0 or hi * ring_scale > 1.0 + 1e-6:
+                raise ValueError('an 8-bit history holds values in [0, 1]')
+        self.values = resolved
+        self.ctx = context or Context.default()
+        self.module = self.ctx.module(self.spec)
+        L = int(steps_per_launch)
+        sd = (SubjectDesc * len(self.subjects))()
+        for k, (csr, p) in enumerate(self.subjects):
+            if csr.n_group != 1:
+                raise ValueError('a subject has one delay row')
+            if L > 1 and csr.nnz and (int(np.min(csr.delays)) < max(L - 1, 1)):
+                raise ValueError('subject %d: %d steps per launch need every '
+                                 'delay >= %d' % (k, L, L - 1))
+            sd[k].n_node, sd[k].n_inst, sd[k].nnz = csr.nnode, p, csr.nnz
+            sd[k].horizon = horizon_for(csr.max_delay, L)
+        self.H = [sd[k].horizon for k in range(len(self.subjects))]
+        counts = self.spec.counts()
+        d = BatchDesc()
+        d.n_subject = len(self.subjects)
+        d.n_svar, d.n_ring, d.n_coupled = self.spec.S, self.spec.R, self.spec.C
+        d.n_pinst, d.n_pnode = counts['inst'], counts['node']
+        d.n_pni, d.n_pglob = counts['ni'], counts['glob']
+        d.dt = scheme.dt_value(dt)
+        d.scheme = 1 if self.spec.em else 0
+        d.waves = d.node_block = int(waves)
+        d.steps_per_launch = L
+        d.ipl = int(ipl)
+        d.ring_type = emit.RING_TYPES[ring_dtype]
+        d.ring_scale = float(ring_scale)
+        self.desc = d
+        self.h = ctypes.c_void_p()
+        check(lib().tvbx_batch_create(self.ctx.h, self.module, ctypes.byref(d),
+                                      sd, ctypes.byref(self.h)))
+        for k, (csr, _) in enumerate(self.subjects):
+            check(lib().tvbx_batch_set_connectome(
+                self.h, k, _ptr(csr.row), _ptr(csr.col), _ptr(csr.weight),
+                _ptr(np.ascontiguousarray(csr.delays[0], np.int32))))
+        self.set_params(resolved)
+
+    def __del__(self):
+        h = getattr(self, 'h', None)
+        if h is not None and h.value:
+            try:
+                lib().tvbx_batch_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    def _default(self, key):
+        if key == 'sigma':
+            return self.scheme.sigma
+        scope, name = key.split(':', 1)
+        return (self.model.const if scope == 'm' else self.cfun.param)[name]
+
+    def set_params(self, values):
+        for k, v in values.items():
+            key = _resolve(self.spec, k) if k != 'sigma' else 'sigma'
+            self.values[key] = v
+        counts = self.spec.counts()
+        Ns = [csr.nnode for csr, _ in self.subjects]
+        arrs = {'glob': np.zeros(counts['glob'], np.float32),
+                'inst': np.zeros((counts['inst'], self.P_total), np.float32),
+                'node': np.zeros((counts['node'], sum(Ns)), np.float32)}
+        if counts['ni']:
+            raise ValueError('batches take no node-and-instance parameters')
+        for key, kind, idx in self.spec.param_layout():
+            v = self.values.get(key, None)
+            if v is None:
+                v = self._default(key)
+            if isinstance(v, E.Expr) or v is None:
+                raise ValueError('parameter %s needs a value' % key)
+            if kind == 'glob':
+                arrs['glob'][idx] = v
+            elif kind == 'inst':
+                arrs['inst'][idx] = np.broadcast_to(np.asarray(v, np.float64),
+                                                    (self.P_total, ))
+            else:
+                if isinstance(v, (list, tuple)):
+                    v = np.concatenate([np.broadcast_to(np.asarray(
+                        x, np.float64), (n, )) for x, n in zip(v, Ns)])
+                arrs['node'][idx] = np.broadcast_to(np.asarray(v), (sum(Ns), ))
+        for kind, code in (('glob', 0), ('inst', 1), ('node', 2)):
+            a = np.ascontiguousarray(arrs[kind])
+            check(lib().tvbx_batch_set_params(self.h, code, _ptr(a), a.size))
+
+    def set_state(self, k, x):
+        "x: (S, N_k, P_k) of subject k"
+        csr, p = self.subjects[k]
+        x = np.ascontiguousarray(x, np.float32)
+        if x.shape != (self.spec.S, csr.nnode, p):
+            raise ValueError('state of subject %d must be %s' %
+                             (k, (self.spec.S, csr.nnode, p)))
+        check(lib().tvbx_batch_set_state(self.h, k, _ptr(x)))
+
+    def get_state(self, k):
+        csr, p = self.subjects[k]
+        x = np.zeros((self.spec.S, csr.nnode, p), np.float32)
+        check(lib().tvbx_batch_get_state(self.h, k, _ptr(x)))
+        return x
+
+    @property
+    def step(self):
+        v = ctypes.c_longlong()
+        check(lib().tvbx_batch_get_step(self.h, ctypes.byref(v)))
+        return v.value
+
+    @step.setter
+    def step(self, n):
+        check(lib().tvbx_batch_set_step(self.h, int(n)))
+
+    def init_history(self, mode='zeros'):
+        check(lib().tvbx_batch_init_history(
+            self.h, {'zeros': 0, 'constant': 1, 'current': 2}[mode]))
+
+    def set_noise(self, seed=None):
+        if seed is None:
+            check(lib().tvbx_batch_set_noise(self.h, 0, 0))
+        else:
+            check(lib().tvbx_batch_set_noise(self.h, 1, int(seed)))
+
+    def run(self, n_steps):
+        check(lib().tvbx_batch_run(self.h, int(n_steps)))
+
+    def sync(self):
+        check(lib().tvbx_batch_sync(self.h))
+
+    def flags(self):
+        f = np.zeros(self.P_total, np.uint32)
+        check(lib().tvbx_batch_get_flags(self.h, _ptr(f)))
+        return f
+
 
 class NetworkStepExecutor:
     """Callable with the arguments of the reference's fused kernel
