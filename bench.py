@@ -11,8 +11,10 @@ timed region; the timed region is K steps bracketed by stream
 synchronisations.  Prints one JSON line.
 
 value        instance*steps/s of the batched mixed-precision sweep at N=998
-             (Jansen-Rit, P=1024, 32 delay bins, bf16 history, FP32 state,
-             FP64 FC): P * K / wall time
+             (Jansen-Rit, P=1024, 32 delay bins, 8-bit fixed-point history of
+             the [0, 1]-scaled sigmoid, FP32 state and sums, FP64 FC):
+             P * K / wall time
+variants     the same sweep with a bfloat16 history (and others on request)
 plain        the same sweep through the plain FP32 kernel (exact delays,
              float32 history); vs_plain = value / plain value
 fc_err_vs_fp64
@@ -203,6 +205,101 @@ def cpu_legs(plain_cfg, fc_idx, n_fc_steps, n_inst, n_steps, threads):
     return base, fc64
 
 
+def bench_batch(args):
+    """The uneven-size sweep: subjects with different connectomes in one
+    launch per block of steps."""
+    import numpy as np
+    from tvb_hpc import autotune, configs
+    cfg = configs.BATCHES[args.workload]()
+    K = args.steps or cfg.n_steps
+    W = args.warmup
+    la = dict(autotune.best_batch_launch(cfg))
+    log('batch', cfg.name, 'subjects', len(cfg.subjects), 'P', cfg.n_inst,
+        'launch', la)
+    b = cfg.batch(**la)
+    for k, x in enumerate(cfg.initial_states()):
+        b.set_state(k, x)
+    b.init_history('zeros')
+    b.set_noise(seed=cfg.seed + 1)
+    b.run(W)
+    b.sync()
+    t0 = time.perf_counter()
+    b.run(K)
+    b.sync()
+    el = time.perf_counter() - t0
+    assert not b.flags().any(), 'non-finite state in the benchmark run'
+    # launch time: back-to-back launches of one block of steps each
+    L = la['steps_per_launch']
+    n_l = max(1, args.profile_steps // L)
+    b.sync()
+    t1 = time.perf_counter()
+    b.run(n_l * L)
+    b.sync()
+    launch_ms = (time.perf_counter() - t1) / n_l * 1e3
+    nnz = sum(c.nnz * p for c, p in cfg.subjects)
+    nodes = sum(c.nnode * p for c, p in cfg.subjects)
+    bytes_step = 4 * (nnz + nodes)      # gathers + ring writes, all subjects
+    achieved = bytes_step * L / (launch_ms * 1e-3) / 1e9
+    Ns = sorted({c.nnode for c, _ in cfg.subjects})
+    out = {
+        'metric': METRIC, 'value': cfg.n_inst * K / el,
+        'unit': 'instance*steps/s', 'n_gpus': 1, 'steps': K, 'warmup': W,
+        'ms_per_step': el / K * 1e3, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {'workload': '%s: %s' % (cfg.name, cfg.description),
+                   'subjects': len(cfg.subjects), 'n_inst': cfg.n_inst,
+                   'node_counts': Ns, 'nnz_total': sum(
+                       c.nnz for c, _ in cfg.subjects),
+                   'dt_ms': cfg.scheme.dt, 'launch': la},
+        'roofline': {'bound': 'hbm', 'achieved': achieved,
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': traffic_from_profile('tvbx_step_batch',
+                                                     args.workload, la),
+                     'kernel': 'tvbx_step_batch',
+                     'bytes_per_step': bytes_step, 'steps_per_launch': L,
+                     'launch_ms': launch_ms,
+                     'launch_time': 'wall time of back-to-back launches'},
+    }
+    del b
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        # float32 oracle on a bounded sample: the first instances of every
+        # subject
+        t = 0.0
+        work = 0
+        n_s = args.cpu_steps
+        off = 0
+        for k, (csr, p) in enumerate(cfg.subjects):
+            idx = np.arange(off, off + min(p, threads))
+            off += p
+            prm = {'J': np.asarray(cfg.params['m:J'])[idx]}
+            ref = O.OracleSim('ReducedWongWang', 'Linear', csr.nnode, csr.row,
+                              csr.col, csr.weight, csr.delays, len(idx),
+                              csr.max_delay + 2, cfg.scheme.dt, scheme='em',
+                              params=prm, cparams={'a': np.asarray(
+                                  cfg.params['c:a'])[idx]},
+                              sigma=np.asarray(cfg.params['sigma'])[idx],
+                              dtype=np.float32, n_threads=threads)
+            ref.set_state(cfg.model.initial_state(
+                csr.nnode, len(idx), np.random.default_rng(k)))
+            ref.init_history('zeros')
+            t1 = time.perf_counter()
+            ref.run(n_s, seed=1)
+            t += time.perf_counter() - t1
+            work += len(idx) * n_s
+        out['cpu_baseline'] = {
+            'value': work / t, 'unit': 'instance*steps/s', 'cores': threads,
+            'kind': 'port',
+            'sample': '%d instances of each of the %d subjects x %d steps on '
+                      '%d threads (float32 oracle, %.1f s)' % (
+                          threads, len(cfg.subjects), n_s, threads, t)}
+    print(json.dumps(out), flush=True)
+
+
 def launch_of(cfg, arg):
     from tvb_hpc import autotune
     if arg:
@@ -216,7 +313,10 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=None)
     ap.add_argument('--warmup', type=int, default=200)
-    ap.add_argument('--workload', default='jansenrit998_bf16')
+    ap.add_argument('--workload', default='jansenrit998_u8')
+    ap.add_argument('--variants', default='jansenrit998_bf16',
+                    help='other mixed-precision workloads timed alongside '
+                         "(comma separated, or 'none')")
     ap.add_argument('--plain', default='jansenrit998',
                     help="plain-kernel workload to compare with, or 'none'")
     ap.add_argument('--no-grad', action='store_true')
@@ -224,7 +324,7 @@ def main():
     ap.add_argument('--fc-instances', type=int, default=8,
                     help='instances whose FC is checked against float64')
     ap.add_argument('--cpu-inst', type=int, default=64)
-    ap.add_argument('--cpu-steps', type=int, default=1000)
+    ap.add_argument('--cpu-steps', type=int, default=2500)
     ap.add_argument('--graph-window', type=int, default=100,
                     help='steps per captured graph window')
     ap.add_argument('--no-graph', action='store_true')
@@ -257,6 +357,8 @@ def main():
     import numpy as np
     from tvb_hpc import configs
 
+    if args.workload in configs.BATCHES:
+        return bench_batch(args)
     gw = 0 if args.no_graph else args.graph_window
     cfg = configs.ALL[args.workload]()
     K = args.steps or cfg.n_steps
@@ -304,6 +406,22 @@ def main():
                         'ms_per_step': plain_r['ms_per_step'],
                         'launch': pl, 'roofline': plain_r['roofline']}
         out['vs_plain'] = main_r['value'] / plain_r['value']
+    variants = {}
+    for vname in ([] if args.variants == 'none' else
+                  args.variants.split(',')):
+        if vname in (args.workload, args.plain):
+            continue
+        vcfg = configs.ALL[vname]()
+        vl = launch_of(vcfg, None)
+        log('variant', vcfg.name, 'launch', vl)
+        vr = time_sweep(vcfg, vl, K, W, gw, args.profile_steps, fc_idx)
+        variants[vname] = {'workload': '%s: %s' % (vcfg.name,
+                                                   vcfg.description),
+                           'value': vr['value'],
+                           'ms_per_step': vr['ms_per_step'], 'launch': vl,
+                           'dtype': 'f32 state, %s history' % vcfg.ring_dtype,
+                           'roofline_frac': vr['roofline']['frac'],
+                           'fc': vr['fc']}
     if not args.no_grad:
         out['grad'] = time_gradient()
     if not args.no_cpu:
@@ -322,7 +440,13 @@ def main():
                        float(np.abs(main_r['fc'] - fc64).max())}
             if plain_r is not None and plain_r['fc'] is not None:
                 err['plain'] = float(np.abs(plain_r['fc'] - fc64).max())
+            for vname, v in variants.items():
+                if v['fc'] is not None:
+                    err[vname] = float(np.abs(v['fc'] - fc64).max())
             out['fc_err_vs_fp64'] = err
+    if variants:
+        out['variants'] = {k: {kk: vv for kk, vv in v.items() if kk != 'fc'}
+                           for k, v in variants.items()}
     print(json.dumps(out), flush=True)
 
 

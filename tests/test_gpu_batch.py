@@ -18,8 +18,8 @@ def _subjects():
     """three connectomes (different N, seeds, speeds) with 64, 100 and 64
     instances: the middle one pads to 128 lanes"""
     out = []
-    for N, seed, v, P in ((12, 3, 2.0, 64), (20, 4, 5.0, 100),
-                          (16, 5, 3.0, 64)):
+    for N, seed, v, P in ((12, 3, 1.0, 64), (20, 4, 2.0, 100),
+                          (16, 5, 1.5, 64)):
         csr = small_csr(N=N, seed=seed, speeds=(v, ), dt=0.1, lmax=50.0)
         out.append((csr, P))
     return out
@@ -32,7 +32,20 @@ def _params(P_total):
             'sigma': rng.uniform(1e-3, 1e-2, P_total).astype(np.float32)}
 
 
+def _block_len(cap=8):
+    "longest temporal block every test subject's delays allow (<= cap)"
+    return min(min(int(csr.delays.min()) for csr, _ in _subjects()) + 1, cap)
+
+
+def _launch(launch):
+    la = dict(launch)
+    if la.get('steps_per_launch') == 'L':
+        la['steps_per_launch'] = _block_len()
+    return la
+
+
 def _batch(launch, em=True):
+    launch = _launch(launch)
     subj = _subjects()
     m = model.ReducedWongWang()
     c = coupling.Linear(m)
@@ -64,8 +77,8 @@ def _inst0(subj, ipl=1):
 
 @pytest.mark.parametrize('launch', [
     dict(waves=4, steps_per_launch=1),
-    dict(waves=2, steps_per_launch=8),
-    dict(waves=1, steps_per_launch=8, min_waves=4)])
+    dict(waves=2, steps_per_launch='L'),
+    dict(waves=1, steps_per_launch='L', min_waves=4)])
 def test_batch_matches_oracle_per_subject(launch):
     """Philox noise: subject k's instances use the counters of their
     batch-wide indices, which the oracle takes as an instance offset."""
@@ -92,7 +105,7 @@ def test_batch_matches_oracle_per_subject(launch):
 def test_batch_equals_one_simulation_per_subject():
     """Without noise each subject of the batch computes exactly what a
     simulation of that subject alone computes with the same launch."""
-    launch = dict(waves=2, steps_per_launch=8)
+    launch = dict(waves=2, steps_per_launch='L')
     b, subj, prm, x0, m, c = _batch(launch, em=False)
     b.run(400)
     off = np.cumsum([0] + [p for _, p in subj])
@@ -101,7 +114,7 @@ def test_batch_equals_one_simulation_per_subject():
         sim = Simulation(m, c, scheme.EulerStep(0.1), csr, p,
                          params={'c:a': prm['c:a'][sl], 'm:J': prm['m:J'][sl]},
                          kinds={'c:a': 'inst', 'm:J': 'inst'}, waves=2,
-                         node_block=2, steps_per_launch=8)
+                         node_block=2, steps_per_launch=_block_len())
         sim.set_state(x0[k])
         sim.init_history('zeros')
         sim.run(400)
@@ -111,7 +124,7 @@ def test_batch_equals_one_simulation_per_subject():
 def test_batch_blocked_equals_single_step_launches():
     bs = []
     for launch in (dict(waves=4, steps_per_launch=1),
-                   dict(waves=4, steps_per_launch=8)):
+                   dict(waves=4, steps_per_launch='L')):
         b, subj, prm, x0, m, c = _batch(launch)
         b.set_noise(seed=3)
         b.run(300)
