@@ -120,39 +120,51 @@ def time_sweep(cfg, launch, K, W, graph_window, profile_steps, fc_idx):
 
 
 def time_gradient(n_evals=2, window=1000):
-    """Gradient evaluations/s of the FC-fit workload: checkpointed forward
-    of all instances, then the adjoint given dL/dsamples."""
+    """Gradient evaluations/s of the FC-fit workload, end to end on the
+    device: checkpointed forward of all instances, FC loss against a
+    target FC (from a run at G = 0.5) in torch, and the adjoint."""
     import numpy as np
+    import torch
+    from tvb_hpc import autograd as AG
     from tvb_hpc import autotune, configs
     cfg = configs.ALL['mpr84']()
     la = dict(autotune.best_launch(cfg))
     T, K = cfg.n_steps, cfg.tavg_period
+    N, P = cfg.csr.nnode, cfg.n_inst
     sim = cfg.simulation(adjoint=True, fc=False, sample_slots=T // K, **la)
     x0 = cfg.initial_state()
-    dS = np.random.default_rng(0).standard_normal(
-        (T // K, cfg.csr.nnode, cfg.n_inst)).astype(np.float32) * 1e-3
+    dev = torch.device('cuda:0')
+    G = torch.tensor(np.asarray(cfg.params['c:a'], np.float32), device=dev)
+    eta = torch.full((N, P), float(cfg.model.const['eta']), device=dev)
+    with torch.no_grad():
+        s_t = AG.integrate(sim, {'c:a': torch.full_like(G, 0.5),
+                                 'm:eta': eta}, x0, T, window, seed=5)
+        fct = AG.fc(s_t).mean(dim=0)
+        del s_t
     times = []
     for e in range(n_evals + 1):
-        sim.set_state(x0)
-        sim.step = 0
-        sim.init_history('zeros')
-        sim.set_noise(seed=5)
-        sim.sync()
+        Gt = G.clone().requires_grad_(True)
+        et = eta.clone().requires_grad_(True)
+        torch.cuda.synchronize()
         t = time.perf_counter()
-        sim.checkpoint_run(T, window)
-        g = sim.adjoint(dS, 0)
-        sim.sync()
+        s = AG.integrate(sim, {'c:a': Gt, 'm:eta': et}, x0, T, window,
+                         seed=5)
+        loss = AG.fc_loss(s, fct).sum()
+        loss.backward()
+        torch.cuda.synchronize()
         times.append(time.perf_counter() - t)
-        assert all(np.isfinite(v).all() for v in g.values())
+        assert torch.isfinite(Gt.grad).all() and torch.isfinite(et.grad).all()
+        del s, loss
     ms = float(np.median(times[1:])) * 1e3
     del sim
     return {'workload': '%s: %s, %d steps, checkpoint every %d' %
                         (cfg.name, cfg.description, T, window),
             'evals_per_s': 1e3 / ms, 'ms_per_eval': ms,
-            'instance_gradients_per_s': cfg.n_inst * 1e3 / ms,
-            'params': sorted(g), 'launch': la,
-            'note': 'one evaluation = forward with checkpoints + replay + '
-                    'reverse sweep for all instances, given dL/dsamples'}
+            'instance_gradients_per_s': P * 1e3 / ms,
+            'params': ['c:a (P)', 'm:eta (N x P)'], 'launch': la,
+            'note': 'one evaluation = forward with checkpoints, FC loss in '
+                    'torch on the device, replay + reverse sweep, for all '
+                    'instances'}
 
 
 def cpu_legs(plain_cfg, fc_idx, n_fc_steps, n_inst, n_steps, threads):

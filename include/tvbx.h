@@ -181,8 +181,8 @@ int tvbx_sim_run(tvbx_sim *sim, long long n_steps, const float *noise,
  * null); requires compat_outputs */
 int tvbx_sim_get_outputs(tvbx_sim *sim, float *input, float *drift,
                          float *diffs);
-/* monitor samples not yet folded into FC: out[count][N][P]; *first is the
- * global sample index of out[0] */
+/* monitor samples not yet folded into FC: out[count][N][P] (host or device
+ * memory); *first is the global sample index of out[0] */
 int tvbx_sim_get_samples(tvbx_sim *sim, float *out, long long *first,
                          int *count);
 /* fold pending samples into FC and write correlations of instances

@@ -183,3 +183,27 @@ def test_adjoint_with_temporal_blocking():
     np.testing.assert_array_equal(out[0][0], out[1][0])
     for k in out[0][1]:
         np.testing.assert_array_equal(out[0][1][k], out[1][1][k])
+
+
+def test_autograd_device_tensors_match_host_tensors():
+    """With CUDA tensors the samples and dL/dsamples stay on the device
+    (strided device copies, the adjoint reading a device pointer); the
+    gradients equal those of the host-tensor path."""
+    csr, m, c, sc, G, eta, x0 = _setup()
+    s_t = _oracle_samples(csr, np.full(P, 0.5), eta, x0)
+    fct = AG.fc(torch.from_numpy(s_t))[0].numpy()
+    grads = []
+    for dev in ('cpu', 'cuda'):
+        sim = _device_sim(csr, m, c, sc, G, eta)
+        Gt = torch.tensor(G, device=dev, requires_grad=True)
+        et = torch.tensor(eta, device=dev, requires_grad=True)
+        smp = AG.integrate(sim, {'c:a': Gt, 'm:eta': et}, x0, T, WIN,
+                           seed=SEED)
+        assert smp.device.type == dev
+        AG.fc_loss(smp, fct).sum().backward()
+        grads.append((Gt.grad.cpu().numpy(), et.grad.cpu().numpy(),
+                      smp.detach().cpu().numpy()))
+    np.testing.assert_array_equal(grads[0][2], grads[1][2])
+    for a, b in zip(grads[0][:2], grads[1][:2]):
+        np.testing.assert_allclose(b, a, rtol=1e-5,
+                                   atol=1e-6 * np.abs(a).max())

@@ -44,6 +44,13 @@ tvbx_adjoint(const tvbx_adj_args A) {
 #pragma unroll
   for (int q = 0; q < TVBX_IPL; ++q) tvbx_load_lane(lp[q], a, pb + q);
 
+  // with one node per wave, its CSC row serves every step of the launch
+  const bool one_node = i_end - i_begin <= nwave;
+  tvbx_row crw;
+  if (one_node && i_begin + wave < i_end)
+    tvbx_load_row_of(ccol, crow, cw, i_begin + wave, crw);
+  const unsigned lane_off = (unsigned)pb * 4u;
+
   for (int k = 0; k < a.n_steps; ++k) {
     const long long n = a.step0 - k;
     const int t = (int)(n % H);
@@ -51,20 +58,62 @@ tvbx_adjoint(const tvbx_adj_args A) {
     for (int j0 = i_begin + wave; j0 < i_end; j0 += nwave) {
       const int j = __builtin_amdgcn_readfirstlane(j0);
       const size_t jp = (size_t)j * a.pitch + pb;
-      // mu of the targets of j's outgoing edges at steps n + d
+      if (!one_node) tvbx_load_row_of(ccol, crow, cw, j, crw);
+      // mu of the targets of j's outgoing edges at steps n + d: edges in
+      // CSC order, indices read back from the row's registers
       float accm[TVBX_IPL];
 #pragma unroll
       for (int q = 0; q < TVBX_IPL; ++q) accm[q] = 0.0f;
-      for (int e = ccol[j]; e < ccol[j + 1]; ++e) {
-        const unsigned c = crow[e];
-        int slot = t + (int)(c >> 16);
-        slot -= (slot >= H) ? H : 0;
-        float v[TVBX_IPL];
-        tvbx_ldf(mu_r + ((size_t)slot * N + (c & 0xffffu)) * a.pitch + pb, v);
-        const float we = cw[e];
+      const int deg = crw.e1 - crw.e0;
+#define TVBX_ADJ_SRC(C)                                                       \
+  reinterpret_cast<const float *>(                                            \
+      reinterpret_cast<const char *>(                                         \
+          mu_r + ((size_t)(t + (int)((C) >> 16) -                             \
+                           ((t + (int)((C) >> 16)) >= H ? H : 0)) * N +       \
+                  ((C) & 0xffffu)) * a.pitch) +                               \
+      tvbx_lane_offset(lane_off))
+// TVBX_UNROLL gathers in flight, accumulated in CSC order
+#define TVBX_ADJ_RANGE(NE, CV, WV)                                            \
+  {                                                                           \
+    int u0 = 0;                                                               \
+    for (; u0 + TVBX_UNROLL <= (NE); u0 += TVBX_UNROLL) {                     \
+      float v[TVBX_UNROLL][TVBX_IPL];                                         \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {               \
+        const unsigned c = tvbx_lane_u((CV), u0 + u);                         \
+        tvbx_ldf(TVBX_ADJ_SRC(c), v[u]);                                      \
+      }                                                                       \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {               \
+        const float we = tvbx_lane_f((WV), u0 + u);                           \
+        _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) accm[q] =        \
+            fmaf(we, v[u][q], accm[q]);                                       \
+      }                                                                       \
+    }                                                                         \
+    for (; u0 < (NE); ++u0) {                                                 \
+      const unsigned c = tvbx_lane_u((CV), u0);                               \
+      float v[TVBX_IPL];                                                      \
+      tvbx_ldf(TVBX_ADJ_SRC(c), v);                                           \
+      const float we = tvbx_lane_f((WV), u0);                                 \
+      _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) accm[q] =          \
+          fmaf(we, v[q], accm[q]);                                            \
+    }                                                                         \
+  }
+      {
+        const int n0 = min(deg, 64);
+        TVBX_ADJ_RANGE(n0, crw.cd0, crw.w0)
+        const int n1 = min(max(deg - 64, 0), 64);
+        if (n1 > 0) TVBX_ADJ_RANGE(n1, crw.cd1, crw.w1)
+        // rows longer than 128 edges (rare): scalar loads
+        for (int e = crw.e0 + 128; e < crw.e1; ++e) {
+          const unsigned c = crow[e];
+          float v[TVBX_IPL];
+          tvbx_ldf(TVBX_ADJ_SRC(c), v);
+          const float we = cw[e];
 #pragma unroll
-        for (int q = 0; q < TVBX_IPL; ++q) accm[q] = fmaf(we, v[q], accm[q]);
+          for (int q = 0; q < TVBX_IPL; ++q) accm[q] = fmaf(we, v[q], accm[q]);
+        }
       }
+#undef TVBX_ADJ_RANGE
+#undef TVBX_ADJ_SRC
       float xs[TVBX_S][TVBX_IPL], ls[TVBX_S][TVBX_IPL], sm[TVBX_IPL];
 #pragma unroll
       for (int s = 0; s < TVBX_S; ++s) {

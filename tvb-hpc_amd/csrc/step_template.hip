@@ -261,17 +261,23 @@ struct tvbx_row {
   float w0, w1;
 };
 
+__device__ __forceinline__ void tvbx_load_row_of(
+    const int *__restrict__ rowptr, const unsigned *__restrict__ cd,
+    const float *__restrict__ w, int i, tvbx_row &r) {
+  const int lane = threadIdx.x & 63;
+  r.e0 = rowptr[i];
+  r.e1 = rowptr[i + 1];
+  const int n = r.e1 - r.e0;
+  r.cd0 = lane < n ? cd[r.e0 + lane] : 0u;
+  r.w0 = lane < n ? w[r.e0 + lane] : 0.0f;
+  r.cd1 = lane + 64 < n ? cd[r.e0 + 64 + lane] : 0u;
+  r.w1 = lane + 64 < n ? w[r.e0 + 64 + lane] : 0.0f;
+}
+
 __device__ __forceinline__ void tvbx_load_row(const tvbx_step_args &a,
                                               const unsigned *__restrict__ cd,
                                               int i, tvbx_row &r) {
-  const int lane = threadIdx.x & 63;
-  r.e0 = a.row[i];
-  r.e1 = a.row[i + 1];
-  const int n = r.e1 - r.e0;
-  r.cd0 = lane < n ? cd[r.e0 + lane] : 0u;
-  r.w0 = lane < n ? a.weight[r.e0 + lane] : 0.0f;
-  r.cd1 = lane + 64 < n ? cd[r.e0 + 64 + lane] : 0u;
-  r.w1 = lane + 64 < n ? a.weight[r.e0 + 64 + lane] : 0.0f;
+  tvbx_load_row_of(a.row, cd, a.weight, i, r);
 }
 
 __device__ __forceinline__ unsigned tvbx_lane_u(unsigned v, int l) {

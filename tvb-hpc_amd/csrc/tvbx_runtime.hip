@@ -1371,12 +1371,21 @@ extern "C" int tvbx_sim_get_samples(tvbx_sim *s, float *out, long long *first,
   if (first) *first = m0;
   if (!out) return TVBX_OK;
   HIPCHK(hipSetDevice(s->ctx->device));
-  for (long long k = 0; k < n; ++k) {
-    const float *src = (const float *)s->samples.p +
-                       (size_t)((m0 + k) % d.sample_slots) * s->NP;
-    int rc = down2d(s, out + (size_t)k * d.n_node * d.n_inst, src, d.n_node);
-    if (rc != TVBX_OK) return rc;
+  // the slots hold a ring of samples: at most two contiguous ranges, each
+  // one strided copy (host or device destination)
+  const size_t P = (size_t)d.n_inst;
+  long long k = 0;
+  while (k < n) {
+    const long long slot = (m0 + k) % d.sample_slots;
+    const long long len = std::min<long long>(n - k, d.sample_slots - slot);
+    HIPCHK(hipMemcpy2DAsync(out + (size_t)k * d.n_node * P, P * 4,
+                            (const float *)s->samples.p + (size_t)slot * s->NP,
+                            (size_t)s->pitch * 4, P * 4,
+                            (size_t)len * d.n_node, hipMemcpyDefault,
+                            s->ctx->stream));
+    k += len;
   }
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
   if (!d.fc) s->samples_used = s->samples_made;
   return TVBX_OK;
 }

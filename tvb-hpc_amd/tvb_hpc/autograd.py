@@ -31,16 +31,33 @@ class _Integrate(torch.autograd.Function):
         sim.init_history('zeros')
         sim.set_noise(seed)
         sim.checkpoint_run(n_steps, window)
-        samples, first = sim.samples()
-        ctx.sim, ctx.keys, ctx.first = sim, keys, first
+        ctx.sim, ctx.keys = sim, keys
         ctx.device = values[0].device
         ctx.dtypes = [v.dtype for v in values]
+        if ctx.device.type == 'cuda':
+            # samples stay on the device: one strided device-to-device copy
+            n, first = sim.pending_samples()
+            out = torch.empty((n, sim.N, sim.P), dtype=torch.float32,
+                              device=ctx.device)
+            sim.samples_into(out.data_ptr())
+            ctx.first = first
+            return out
+        samples, first = sim.samples()
+        ctx.first = first
         return torch.from_numpy(samples).to(ctx.device)
 
     @staticmethod
     def backward(ctx, d_samples):
-        g = ctx.sim.adjoint(d_samples.detach().float().cpu().numpy(),
-                            ctx.first)
+        if d_samples.is_cuda:
+            # dL/dsamples stays on the device; torch's stream must have
+            # finished writing it before the simulation's stream reads it
+            ds = d_samples.detach().float().contiguous()
+            torch.cuda.current_stream(ds.device).synchronize()
+            g = ctx.sim.adjoint(ds.data_ptr(), ctx.first,
+                                n_samples=ds.shape[0])
+        else:
+            g = ctx.sim.adjoint(d_samples.detach().float().cpu().numpy(),
+                                ctx.first)
         grads = [torch.from_numpy(np.asarray(g[k])).to(ctx.device, dt)
                  for k, dt in zip(ctx.keys, ctx.dtypes)]
         return (None, None, None, None, None, None) + tuple(grads)
@@ -58,7 +75,8 @@ def integrate(sim, params, x0, n_steps, window, seed=None):
 def fc(samples):
     """Pearson FC per instance: samples (M, N, P) -> (P, N, N)."""
     x = samples - samples.mean(dim=0, keepdim=True)
-    cov = torch.einsum('mip,mjp->pij', x, x)
+    xp = x.permute(2, 0, 1).contiguous()              # (P, M, N)
+    cov = torch.bmm(xp.transpose(1, 2).contiguous(), xp)
     d = torch.sqrt(torch.diagonal(cov, dim1=1, dim2=2))
     return cov / (d[:, :, None] * d[:, None, :])
 

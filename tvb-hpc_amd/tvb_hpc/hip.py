@@ -524,21 +524,28 @@ class Simulation:
         check(lib().tvbx_sim_checkpoint_run(self.h, int(n_steps),
                                             int(window)))
 
-    def adjoint(self, dsamples, sample0=0):
-        """Gradients of a loss from dL/dsample ((n, N, P), numpy) of the
-        samples produced by the last ``checkpoint_run``.  Returns
+    def adjoint(self, dsamples, sample0=0, n_samples=None):
+        """Gradients of a loss from dL/dsample of the samples produced by
+        the last ``checkpoint_run``: a (n, N, P) float32 numpy array, or the
+        device address of one (an int, e.g. a torch tensor's data_ptr(),
+        with ``n_samples``; the caller has finished writing it).  Returns
         {param key: gradient}: coupling parameters (P,), model parameters
         of kind 'ni' (N, P) and 'inst' (P,)."""
         gc, gm = self.spec.grad_params()
-        ds = np.ascontiguousarray(dsamples, np.float32)
-        n = ds.shape[0]
-        if ds.shape[1:] != (self.N, self.P):
-            raise ValueError('dsamples must be (n, N, P)')
+        if isinstance(dsamples, int):
+            if n_samples is None:
+                raise ValueError('a device address needs n_samples')
+            ptr, on_dev, n, keep = dsamples, 1, int(n_samples), None
+        else:
+            keep = np.ascontiguousarray(dsamples, np.float32)
+            if keep.shape[1:] != (self.N, self.P):
+                raise ValueError('dsamples must be (n, N, P)')
+            ptr, on_dev, n = _ptr(keep), 0, keep.shape[0]
         g_c = np.zeros((max(len(gc), 1), self.P), np.float32)
         g_m = np.zeros((max(len(gm), 1), self.N, self.P), np.float32)
-        check(lib().tvbx_sim_adjoint(self.h, _ptr(ds), 0, int(sample0), n,
-                                     len(gc), _ptr(g_c), len(gm), _ptr(g_m),
-                                     0))
+        check(lib().tvbx_sim_adjoint(self.h, ctypes.c_void_p(ptr), on_dev,
+                                     int(sample0), n, len(gc), _ptr(g_c),
+                                     len(gm), _ptr(g_m), 0))
         out = {}
         for k, key in enumerate(gc):
             out[key] = g_c[k]
@@ -554,6 +561,23 @@ class Simulation:
         df = np.zeros((S, N, P), np.float32)
         check(lib().tvbx_sim_get_outputs(self.h, _ptr(i), _ptr(dr), _ptr(df)))
         return i, dr, df
+
+    def samples_into(self, ptr):
+        """Copy the pending monitor samples to device (or host) memory at
+        address ``ptr`` ((n, N, P) float32, e.g. a torch tensor's
+        data_ptr()); returns (n, index of the first sample)."""
+        cnt, first = ctypes.c_int(), ctypes.c_longlong()
+        check(lib().tvbx_sim_get_samples(self.h, ctypes.c_void_p(ptr),
+                                         ctypes.byref(first),
+                                         ctypes.byref(cnt)))
+        return cnt.value, first.value
+
+    def pending_samples(self):
+        "(number of pending monitor samples, index of the first)"
+        cnt, first = ctypes.c_int(), ctypes.c_longlong()
+        check(lib().tvbx_sim_get_samples(self.h, None, ctypes.byref(first),
+                                         ctypes.byref(cnt)))
+        return cnt.value, first.value
 
     def samples(self):
         "pending monitor samples (n, N, P) and the index of the first"
