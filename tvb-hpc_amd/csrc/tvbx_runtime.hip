@@ -411,7 +411,7 @@ __host__ __device__ inline size_t tri_index(int i, int j, int N) {
   return (size_t)i * N - (size_t)i * (i - 1) / 2 + (size_t)(j - i);
 }
 
-#define FC_TB 8   // node tile edge of the co-moment kernel
+#define FC_TB 8   // edge of one wave's block of node pairs
 #define FC_ST 8   // tiles per super-tile edge (rows kept in one XCD's L2)
 
 // Merge of a block of m = `count` monitor samples (slots first, first + 1,
@@ -447,29 +447,38 @@ __global__ void k_fc_prep(const float *__restrict__ samples, int slots,
   u[ip] = ((double)sh + s / (double)count) - ma;
 }
 
-// co-moments of one 8 x 8 tile of node pairs (I <= J) for the 64 instances
-// of one lane block.  Workgroups are numbered so that each of the 8 XCDs
+// co-moments of one 16 x 16 tile of node pairs (tile I <= tile J) for the
+// 64 instances of one lane block.  Chunks of FC_CHUNK samples of the tile's
+// 32 rows are staged in LDS by all 256 threads (many independent loads in
+// flight); each of the 4 waves then accumulates an 8 x 8 block of FP64
+// co-moments from LDS (the block strictly below the diagonal of a diagonal
+// tile is skipped).  Workgroups are numbered so that each of the 8 XCDs
 // (which take workgroup ids round robin) works through a contiguous range
-// of (instance block, tile), tiles in super-tile order: the sample rows of
-// a super-tile stay in that XCD's L2 while its tiles run.
-__global__ void __launch_bounds__(64)
+// of (instance block, tile), tiles in super-tile order.
+#define FC_TILE 16
+#define FC_CHUNK 8
+__global__ void __launch_bounds__(256)
 k_fc_update(const float *__restrict__ samples, int slots, long long first,
             int count, int N, int pitch, double n_prev,
             const int2 *__restrict__ tiles, int n_tiles, int n_pblk,
             const float *__restrict__ shift, const double *__restrict__ s1,
             const double *__restrict__ u, double *__restrict__ m2) {
+  __shared__ float xs[FC_CHUNK][2 * FC_TILE][64];
   const unsigned per = gridDim.x / 8;  // grid is a multiple of 8
   const unsigned L = (blockIdx.x % 8) * per + blockIdx.x / 8;
-  if (L >= (unsigned)n_tiles * (unsigned)n_pblk) return;
+  if (L >= (unsigned)n_tiles * (unsigned)n_pblk) return;  // whole workgroup
   const int pblk = L / n_tiles;
   const int2 tj = tiles[L % n_tiles];
-  const int p = pblk * 64 + threadIdx.x;
-  const int I0 = tj.x * FC_TB, J0 = tj.y * FC_TB;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p = pblk * 64 + lane;
+  const int I0 = tj.x * FC_TILE, J0 = tj.y * FC_TILE;
+  const int ra = (wave >> 1) * FC_TB, rb = (wave & 1) * FC_TB;
+  const bool active = !(tj.x == tj.y && ra > rb);
   const size_t NP = (size_t)N * pitch;
   float shi[FC_TB], shj[FC_TB];
 #pragma unroll
   for (int a = 0; a < FC_TB; ++a) {
-    const int i = I0 + a, j = J0 + a;
+    const int i = I0 + ra + a, j = J0 + rb + a;
     shi[a] = i < N ? shift[(size_t)i * pitch + p] : 0.0f;
     shj[a] = j < N ? shift[(size_t)j * pitch + p] : 0.0f;
   }
@@ -478,54 +487,47 @@ k_fc_update(const float *__restrict__ samples, int slots, long long first,
   for (int a = 0; a < FC_TB; ++a)
 #pragma unroll
     for (int b = 0; b < FC_TB; ++b) c[a][b] = 0.0;
-  // one sample of prefetch
-  float xi[FC_TB], xj[FC_TB];
-  {
-    const float *x = samples + (size_t)(first % slots) * NP + p;
+  for (int k0 = 0; k0 < count; k0 += FC_CHUNK) {
+    const int kc = min(FC_CHUNK, count - k0);
+    __syncthreads();  // the previous chunk is consumed
+    // LDS row r: node I0 + r (r < 16) or J0 + r - 16; thread rows wave + 4m
+    for (int kk = 0; kk < kc; ++kk) {
+      const float *x = samples + (size_t)((first + k0 + kk) % slots) * NP + p;
 #pragma unroll
-    for (int a = 0; a < FC_TB; ++a) {
-      xi[a] = I0 + a < N ? x[(size_t)(I0 + a) * pitch] : 0.0f;
-      xj[a] = J0 + a < N ? x[(size_t)(J0 + a) * pitch] : 0.0f;
-    }
-  }
-  for (int k = 0; k < count; ++k) {
-    float ni[FC_TB], nj[FC_TB];
-    if (k + 1 < count) {
-      const float *x = samples + (size_t)((first + k + 1) % slots) * NP + p;
-#pragma unroll
-      for (int a = 0; a < FC_TB; ++a) {
-        ni[a] = I0 + a < N ? x[(size_t)(I0 + a) * pitch] : 0.0f;
-        nj[a] = J0 + a < N ? x[(size_t)(J0 + a) * pitch] : 0.0f;
+      for (int m = 0; m < 2 * FC_TILE / 4; ++m) {
+        const int r = wave + 4 * m;
+        const int node = r < FC_TILE ? I0 + r : J0 + r - FC_TILE;
+        xs[kk][r][lane] = node < N ? x[(size_t)node * pitch] : 0.0f;
       }
     }
-    double dj[FC_TB];
+    __syncthreads();
+    if (active) {
+      for (int kk = 0; kk < kc; ++kk) {
+        double dj[FC_TB];
 #pragma unroll
-    for (int b = 0; b < FC_TB; ++b) dj[b] = (double)xj[b] - (double)shj[b];
+        for (int b = 0; b < FC_TB; ++b)
+          dj[b] = (double)xs[kk][FC_TILE + rb + b][lane] - (double)shj[b];
 #pragma unroll
-    for (int a = 0; a < FC_TB; ++a) {
-      const double di = (double)xi[a] - (double)shi[a];
+        for (int a = 0; a < FC_TB; ++a) {
+          const double di = (double)xs[kk][ra + a][lane] - (double)shi[a];
 #pragma unroll
-      for (int b = 0; b < FC_TB; ++b) c[a][b] = fma(di, dj[b], c[a][b]);
-    }
-    if (k + 1 < count) {
-#pragma unroll
-      for (int a = 0; a < FC_TB; ++a) {
-        xi[a] = ni[a];
-        xj[a] = nj[a];
+          for (int b = 0; b < FC_TB; ++b) c[a][b] = fma(di, dj[b], c[a][b]);
+        }
       }
     }
   }
+  if (!active) return;
   const double m = (double)count;
   const double w = n_prev * m / (n_prev + m);
 #pragma unroll
   for (int a = 0; a < FC_TB; ++a) {
-    const int i = I0 + a;
+    const int i = I0 + ra + a;
     if (i >= N) continue;
     const double s1i = s1[(size_t)i * pitch + p];
     const double ui = u[(size_t)i * pitch + p];
 #pragma unroll
     for (int b = 0; b < FC_TB; ++b) {
-      const int j = J0 + b;
+      const int j = J0 + rb + b;
       if (j >= N || j < i) continue;
       const double s1j = s1[(size_t)j * pitch + p];
       const double uj = u[(size_t)j * pitch + p];
@@ -1092,7 +1094,7 @@ static int fc_flush(tvbx_sim *s) {
   long long pending = s->samples_made - s->samples_used;
   if (!d.fc || pending <= 0) return TVBX_OK;
   const int N = d.n_node;
-  const int T = (N + FC_TB - 1) / FC_TB;
+  const int T = (N + FC_TILE - 1) / FC_TILE;
   const int n_tiles = T * (T + 1) / 2;
   int rc = TVBX_OK;
   if (!s->fc_tiles.p) {
@@ -1119,7 +1121,7 @@ static int fc_flush(tvbx_sim *s) {
   const unsigned long long nwg =
       ((unsigned long long)n_tiles * n_pblk + 7) / 8 * 8;
   CHECK(nwg < (1ull << 31), "FC problem too large for one launch");
-  hipLaunchKernelGGL(k_fc_update, dim3((unsigned)nwg), dim3(64), 0,
+  hipLaunchKernelGGL(k_fc_update, dim3((unsigned)nwg), dim3(256), 0,
                      s->ctx->stream, (const float *)s->samples.p,
                      d.sample_slots, first, (int)pending, N, s->pitch, n_prev,
                      (const int2 *)s->fc_tiles.p, n_tiles, n_pblk,

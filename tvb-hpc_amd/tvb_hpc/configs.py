@@ -50,6 +50,9 @@ class SweepConfig:
     ring_dtype: str = 'f32'
     ring_scale: float = 1.0
     delay_bins: Optional[int] = None
+    # monitor samples buffered on the device between FC merges (each merge
+    # reads and writes the FP64 co-moment triangle once)
+    sample_slots: int = 256
 
     @property
     def horizon(self):
@@ -89,7 +92,8 @@ class SweepConfig:
         opts = dict(params=self.params, kinds=self.kinds,
                     group_size=self.group_size, tavg_period=self.tavg_period,
                     bold=self.bold, fc=self.fc, bold_obs=self.bold_obs,
-                    ring_dtype=self.ring_dtype, ring_scale=self.ring_scale)
+                    ring_dtype=self.ring_dtype, ring_scale=self.ring_scale,
+                    sample_slots=self.sample_slots)
         opts.update(kw)
         return Simulation(self.model, self.cfun, self.scheme, self.csr,
                           self.n_inst, **opts)
