@@ -129,6 +129,17 @@ _lib = None
 _lib_lock = threading.Lock()
 
 
+def _one_hip_runtime():
+    """PyTorch-ROCm ships its own HIP and HSA runtimes (same sonames as
+    /opt/rocm's).  Loaded after libtvbx they would be a second runtime in the
+    process, which then finds no GPU; loaded first, libtvbx binds to them.
+    So load PyTorch's, when it is installed, before libtvbx."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Load libtvbx.so (never builds; run ``tvb_hpc.build`` first)."""
     global _lib
@@ -137,6 +148,7 @@ def lib():
             if not os.path.exists(LIB_PATH):
                 raise TvbxError('libtvbx.so not built (%s); run '
                                 'python -m tvb_hpc.build' % LIB_PATH)
+            _one_hip_runtime()
             L = ctypes.CDLL(LIB_PATH)
             for name, args in _SIGS.items():
                 fn = getattr(L, name)
