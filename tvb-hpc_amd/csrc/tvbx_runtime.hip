@@ -457,6 +457,7 @@ __global__ void k_fc_prep(const float *__restrict__ samples, int slots,
 // of (instance block, tile), tiles in super-tile order.
 #define FC_TILE 16
 #define FC_CHUNK 8
+typedef __attribute__((address_space(3))) void tvbx_lds_void;
 __global__ void __launch_bounds__(256)
 k_fc_update(const float *__restrict__ samples, int slots, long long first,
             int count, int N, int pitch, double n_prev,
@@ -487,19 +488,30 @@ k_fc_update(const float *__restrict__ samples, int slots, long long first,
   for (int a = 0; a < FC_TB; ++a)
 #pragma unroll
     for (int b = 0; b < FC_TB; ++b) c[a][b] = 0.0;
+  const int slot0 = (int)(first % slots);
   for (int k0 = 0; k0 < count; k0 += FC_CHUNK) {
     const int kc = min(FC_CHUNK, count - k0);
     __syncthreads();  // the previous chunk is consumed
-    // LDS row r: node I0 + r (r < 16) or J0 + r - 16; thread rows wave + 4m
+    // LDS row r: node I0 + r (r < 16) or J0 + r - 16; wave w stages rows
+    // w + 4m straight from global memory into LDS (lane l's dword lands at
+    // the row's LDS base + 4 l), all of the chunk's loads in flight at once
     for (int kk = 0; kk < kc; ++kk) {
-      const float *x = samples + (size_t)((first + k0 + kk) % slots) * NP + p;
+      int slot = slot0 + k0 + kk;
+      slot -= slot >= slots ? slots : 0;
+      const float *x = samples + (size_t)slot * NP + (size_t)pblk * 64;
 #pragma unroll
       for (int m = 0; m < 2 * FC_TILE / 4; ++m) {
         const int r = wave + 4 * m;
         const int node = r < FC_TILE ? I0 + r : J0 + r - FC_TILE;
-        xs[kk][r][lane] = node < N ? x[(size_t)node * pitch] : 0.0f;
+        if (node < N)
+          __builtin_amdgcn_global_load_lds(x + (size_t)node * pitch + lane,
+                                           (tvbx_lds_void *)&xs[kk][r][0], 4,
+                                           0, 0);
+        else
+          xs[kk][r][lane] = 0.0f;
       }
     }
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's loads have landed
     __syncthreads();
     if (active) {
       for (int kk = 0; kk < kc; ++kk) {
