@@ -177,29 +177,47 @@ def cpu_legs(plain_cfg, fc_idx, n_fc_steps, n_inst, n_steps, threads):
 
     def oracle(cfg, idx, dtype):
         csr = cfg.csr
+        # delay group of every sampled instance (one group per instance)
+        gs = cfg.group_size or cfg.n_inst
+        delays = csr.delays[np.asarray(idx) // gs]
         mp, cp = {}, {}
         for k, v in cfg.params.items():
-            scope, nm = k.split(':', 1)
+            if k == 'sigma':
+                continue
+            if ':' in k:
+                scope, nm = k.split(':', 1)
+            else:
+                scope, nm = ('c' if k in cfg.cfun.param else 'm'), k
             a = np.asarray(v)
             val = a[..., idx] if a.ndim >= 1 and a.shape[-1] == cfg.n_inst \
                 else a
             (mp if scope == 'm' else cp)[nm] = val
         cp.update({k: v for k, v in cfg.cfun.param.items()
                    if k not in cp and not hasattr(v, 'name')})
+        sigma = getattr(cfg.scheme, 'sigma', None)
+        if sigma is not None and 'sigma' in cfg.params:
+            sigma = np.asarray(cfg.params['sigma'])[idx]
         ref = O.OracleSim(type(cfg.model).__name__, type(cfg.cfun).__name__,
-                          csr.nnode, csr.row, csr.col, csr.weight, csr.delays,
+                          csr.nnode, csr.row, csr.col, csr.weight, delays,
                           len(idx), csr.max_delay + 2, cfg.scheme.dt,
-                          scheme='euler', params=mp, cparams=cp, dtype=dtype,
-                          tavg_period=cfg.tavg_period, n_threads=threads)
+                          scheme='euler' if sigma is None else 'em',
+                          params=mp, cparams=cp, sigma=sigma, group_size=1,
+                          dtype=dtype, tavg_period=cfg.tavg_period,
+                          bold=cfg.bold, bold_obs=cfg.bold_obs,
+                          n_threads=threads)
         ref.set_state(cfg.initial_state()[..., idx])
         ref.init_history('zeros')
         return ref
 
+    seed = None
+    if getattr(plain_cfg.scheme, 'sigma', None) is not None:
+        seed = plain_cfg.seed + 1
+
     idx = np.arange(n_inst)
     ref = oracle(plain_cfg, idx, np.float32)
-    ref.run(5)
+    ref.run(5, seed=seed)
     t = time.perf_counter()
-    ref.run(n_steps)
+    ref.run(n_steps, seed=seed)
     el = time.perf_counter() - t
     base = {'value': n_inst * n_steps / el, 'unit': 'instance*steps/s',
             'cores': threads, 'kind': 'port',
@@ -210,7 +228,7 @@ def cpu_legs(plain_cfg, fc_idx, n_fc_steps, n_inst, n_steps, threads):
     if len(fc_idx):
         ref = oracle(plain_cfg, np.asarray(fc_idx), np.float64)
         t = time.perf_counter()
-        s = ref.run(n_fc_steps)['samples']
+        s = ref.run(n_fc_steps, seed=seed)['samples']
         log('float64 oracle FC reference: %d instances x %d steps, %.1f s' %
             (len(fc_idx), n_fc_steps, time.perf_counter() - t))
         fc64 = np.stack([np.corrcoef(s[:, :, k].T) for k in range(len(fc_idx))])
@@ -376,8 +394,12 @@ def main():
     K = args.steps or cfg.n_steps
     W = args.warmup
     N, P, nnz = cfg.csr.nnode, cfg.n_inst, cfg.csr.nnz
+    # FC of a deterministic sweep is compared with float64 (a stochastic
+    # sweep's sample paths separate from the float64 ones)
+    deterministic = getattr(cfg.scheme, 'sigma', None) is None
     fc_idx = np.linspace(0, P - 1, args.fc_instances).astype(int) \
-        if cfg.fc and args.fc_instances > 0 else np.zeros(0, int)
+        if cfg.fc and deterministic and args.fc_instances > 0 \
+        else np.zeros(0, int)
     launch = launch_of(cfg, args.launch)
     log('workload', cfg.name, 'N', N, 'P', P, 'nnz', nnz, 'launch', launch)
     main_r = time_sweep(cfg, launch, K, W, gw, args.profile_steps, fc_idx)
