@@ -288,6 +288,69 @@ __device__ __forceinline__ float tvbx_lane_f(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
+#if !TVBX_ORDER_REF
+// Ring row (slot * R * N + column, coupled observable 0) that the edge
+// packed in cdw reads at ring slot t; evaluated for the 64 edges a vector
+// register holds at once, once per step, so that each gather needs only
+// its wave-uniform row read back with v_readlane.
+__device__ __forceinline__ unsigned tvbx_edge_row(unsigned cdw, int t,
+                                                  const tvbx_step_args &a) {
+  int slot = t - (int)(cdw >> 16);
+  slot += slot < 0 ? a.horizon : 0;
+  return (unsigned)slot * (unsigned)(TVBX_R * a.n_node) + (cdw & 0xffffu);
+}
+
+// Raw buffer descriptor over a ring of less than 4 GiB: a gather is one
+// buffer_load with the lane's byte offset in a vector register and the
+// edge's wave-uniform byte offset as the scalar offset (no scalar address
+// arithmetic per edge).  Word 3 = 0x00020000: 32-bit data format, no
+// swizzle; num_records = 2^32 - 1 (no range clamping).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t
+tvbx_ring_rsrc(const void *ring) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(ring), (short)0,
+                                           (int)0xffffffff, 0x00020000);
+}
+
+__device__ __forceinline__ tvbx_raw tvbx_bload(__amdgpu_buffer_rsrc_t r,
+                                               unsigned voff, unsigned soff) {
+  typedef unsigned tvbx_u2 __attribute__((ext_vector_type(2)));
+  typedef unsigned tvbx_u4 __attribute__((ext_vector_type(4)));
+#if TVBX_RING_T == 0
+#define TVBX_RAW_BYTES (4 * TVBX_IPL)
+#elif TVBX_RING_T == 3
+#define TVBX_RAW_BYTES TVBX_IPL
+#else
+#define TVBX_RAW_BYTES (2 * TVBX_IPL)
+#endif
+#if TVBX_RAW_BYTES == 1
+  return __builtin_bit_cast(
+      tvbx_raw, __builtin_amdgcn_raw_buffer_load_b8(r, voff, soff, 0));
+#elif TVBX_RAW_BYTES == 2
+  return __builtin_bit_cast(
+      tvbx_raw, __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+#elif TVBX_RAW_BYTES == 4
+  return __builtin_bit_cast(
+      tvbx_raw, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+#elif TVBX_RAW_BYTES == 8
+  return __builtin_bit_cast(
+      tvbx_raw, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+#else
+  return __builtin_bit_cast(
+      tvbx_raw, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+#endif
+#undef TVBX_RAW_BYTES
+}
+
+// The same gather for a ring of 4 GiB or more: the row index times the
+// row's bytes (64-bit, scalar) plus the lane's offset.
+__device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
+                                                 unsigned row, size_t rowb,
+                                                 unsigned lane_off) {
+  return *reinterpret_cast<const tvbx_raw *>(ring + (size_t)row * rowb +
+                                             tvbx_lane_offset(lane_off));
+}
+#endif
+
 // sum over edges [b, e) of the row whose packed (delay, column) and weight
 // for edge b + u come from cdw(u), into acc
 #define TVBX_GATHER_RANGE(NE, CDW, WW)                                        \
@@ -320,6 +383,43 @@ __device__ __forceinline__ float tvbx_lane_f(float v, int l) {
     }                                                                       \
   }
 
+#if !TVBX_ORDER_REF
+// sum over the row's edges [0, NE): edge u's wave-uniform address word is
+// IX(u), its gather LD(word) and its weight WW(u).  The groups of
+// TVBX_UNROLL gathers are fenced off from the optimiser's scheduler: all
+// address words first (the v_readlane results then need no wait states
+// before the loads read them), then every load of the group, then the
+// arithmetic, so that a group's loads are all in flight together.
+#define TVBX_GATHER_LD(NE, IX, LD, WW)                                        \
+  {                                                                         \
+    int u0 = 0;                                                             \
+    for (; u0 + TVBX_UNROLL <= (NE); u0 += TVBX_UNROLL) {                   \
+      unsigned sx[TVBX_UNROLL];                                             \
+      tvbx_raw v[TVBX_UNROLL];                                              \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) sx[u] =       \
+          IX(u0 + u);                                                       \
+      __builtin_amdgcn_sched_barrier(0);                                    \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) v[u] =        \
+          LD(sx[u]);                                                        \
+      __builtin_amdgcn_sched_barrier(0);                                    \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {             \
+        const float wu = WW(u0 + u);                                        \
+        float f[TVBX_IPL];                                                  \
+        tvbx_unpack(v[u], f);                                               \
+        _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc[q] =       \
+            fmaf(wu, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);        \
+      }                                                                     \
+    }                                                                       \
+    for (; u0 < (NE); ++u0) {                                               \
+      float f[TVBX_IPL];                                                    \
+      tvbx_unpack(LD(IX(u0)), f);                                           \
+      const float we = WW(u0);                                              \
+      _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc[q] =         \
+          fmaf(we, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);          \
+    }                                                                       \
+  }
+#endif
+
 // Coupling sums c[q][kk] of node i at step n (ring slot t): the CSR row
 // reduction of network.py:115-136 over the edges in row order, with rec_sum
 // recording for the adjoint.
@@ -336,6 +436,18 @@ __device__ __forceinline__ void tvbx_coupling(
   const int n0 = min(deg, 64);
   const int n1 = min(max(deg - 64, 0), 64);
   const int eb = __builtin_amdgcn_readfirstlane(row.e0 + 128);
+#if !TVBX_ORDER_REF
+  // this step's ring rows of the first 128 edges; as byte offsets when the
+  // ring is addressed through a buffer descriptor
+  const size_t rowb = (size_t)a.pitch * sizeof(tvbx_rt);
+  unsigned ro0 = tvbx_edge_row(row.cd0, t, a);
+  unsigned ro1 = tvbx_edge_row(row.cd1, t, a);
+  const __amdgpu_buffer_rsrc_t rs = tvbx_ring_rsrc(ring);
+  if (a.ring32) {
+    ro0 *= (unsigned)rowb;
+    ro1 *= (unsigned)rowb;
+  }
+#endif
 #pragma unroll
   for (int kk = 0; kk < TVBX_C; ++kk) {
     float post[TVBX_IPL], acc[TVBX_IPL];
@@ -357,8 +469,31 @@ __device__ __forceinline__ void tvbx_coupling(
 #define TVBX_W1(x) tvbx_lane_f(row.w1, (x))
 #define TVBX_CDS(x) cd[eb + (x)]
 #define TVBX_WS(x) wt[eb + (x)]
+#if TVBX_ORDER_REF
     TVBX_GATHER_RANGE(n0, TVBX_CD0, TVBX_W0)
     if (n1 > 0) TVBX_GATHER_RANGE(n1, TVBX_CD1, TVBX_W1)
+#else
+    if (a.ring32) {
+      // coupled observable kk: kk * N rows further into the slot
+      const unsigned voff =
+          lane_off + (unsigned)(kk * a.n_node) * (unsigned)rowb;
+#define TVBX_X0(x) tvbx_lane_u(ro0, (x))
+#define TVBX_X1(x) tvbx_lane_u(ro1, (x))
+#define TVBX_LB(o) tvbx_bload(rs, voff, (o))
+      TVBX_GATHER_LD(n0, TVBX_X0, TVBX_LB, TVBX_W0)
+      if (n1 > 0) TVBX_GATHER_LD(n1, TVBX_X1, TVBX_LB, TVBX_W1)
+#undef TVBX_LB
+    } else {
+      const char *rk = reinterpret_cast<const char *>(ring) +
+                       (size_t)kk * a.n_node * rowb;
+#define TVBX_LG(o) tvbx_rowload(rk, (o), rowb, lane_off)
+      TVBX_GATHER_LD(n0, TVBX_X0, TVBX_LG, TVBX_W0)
+      if (n1 > 0) TVBX_GATHER_LD(n1, TVBX_X1, TVBX_LG, TVBX_W1)
+#undef TVBX_LG
+#undef TVBX_X0
+#undef TVBX_X1
+    }
+#endif
     // rows longer than 128 edges (rare): one edge at a time, scalar loads
     for (int u0 = 0; u0 < deg - 128; ++u0) {
       const unsigned w = TVBX_CDS(u0);

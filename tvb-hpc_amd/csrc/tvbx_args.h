@@ -100,6 +100,9 @@ typedef struct tvbx_step_args {
    * pitch */
   int inst_off;
   int param_pitch;
+  /* 1: the ring is smaller than 4 GiB, so gathers address it with 32-bit
+   * byte offsets from one buffer descriptor */
+  int ring32;
 } tvbx_step_args;
 
 /* Argument block of the batched step kernel: per subject (one connectome

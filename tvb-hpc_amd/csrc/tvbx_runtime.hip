@@ -697,6 +697,7 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
     const long long gy = (d.n_node + d.node_block - 1) / d.node_block;
     a.xcd_map = d.xcd_map && (gx * gy) % 8 == 0;
   }
+  a.ring32 = (double)d.horizon * d.n_ring * s->NP * s->esz < 4294967296.0;
   a.ring_unscale = d.ring_type == 0   ? 1.0f
                    : d.ring_type == 3 ? 1.0f / (255.0f * d.ring_scale)
                                       : 1.0f / d.ring_scale;
@@ -1852,6 +1853,8 @@ static void batch_args(tvbx_batch *b, int k, tvbx_step_args &a) {
                                       : 1.0f / d.ring_scale;
   a.inst_off = (int)b->inst0[k];
   a.param_pitch = (int)b->P_pitch;
+  a.ring32 = (double)u.horizon * d.n_ring * u.n_node * b->pitch[k] * b->esz <
+             4294967296.0;
 }
 
 static int batch_upload_args(tvbx_batch *b) {
