@@ -180,6 +180,8 @@ def cpu_legs(plain_cfg, fc_idx, n_fc_steps, n_inst, n_steps, threads):
         # delay group of every sampled instance (one group per instance)
         gs = cfg.group_size or cfg.n_inst
         delays = csr.delays[np.asarray(idx) // gs]
+        weight = csr.weight[np.asarray(idx) // gs] if csr.weight_groups \
+            else csr.weight
         mp, cp = {}, {}
         for k, v in cfg.params.items():
             if k == 'sigma':
@@ -198,7 +200,7 @@ def cpu_legs(plain_cfg, fc_idx, n_fc_steps, n_inst, n_steps, threads):
         if sigma is not None and 'sigma' in cfg.params:
             sigma = np.asarray(cfg.params['sigma'])[idx]
         ref = O.OracleSim(type(cfg.model).__name__, type(cfg.cfun).__name__,
-                          csr.nnode, csr.row, csr.col, csr.weight, delays,
+                          csr.nnode, csr.row, csr.col, weight, delays,
                           len(idx), csr.max_delay + 2, cfg.scheme.dt,
                           scheme='euler' if sigma is None else 'em',
                           params=mp, cparams=cp, sigma=sigma, group_size=1,
@@ -233,6 +235,86 @@ def cpu_legs(plain_cfg, fc_idx, n_fc_steps, n_inst, n_steps, threads):
             (len(fc_idx), n_fc_steps, time.perf_counter() - t))
         fc64 = np.stack([np.corrcoef(s[:, :, k].T) for k in range(len(fc_idx))])
     return base, fc64
+
+
+RMAP_WORKLOADS = {
+    # surface vertices (the reference's 515,056-vertex surface), regions,
+    # state variables, instances
+    'rmap515k': (515056, 400, 2, 64),
+}
+
+
+def bench_rmap(args):
+    """Region mapping of a 515k-vertex surface: to_avg (region sums) and
+    from_avg (vertex gather), device-resident arrays, K calls each."""
+    import ctypes
+    import numpy as np
+    import torch
+    from tvb_hpc import hip
+    nnode, nroi, nvar, P = RMAP_WORKLOADS[args.workload]
+    K = args.steps or 200
+    W = args.warmup
+    rng = np.random.default_rng(5)
+    rmap = rng.integers(0, nroi, size=nnode).astype(np.int32)
+    node_h = rng.standard_normal((nnode, nvar, P)).astype(np.float32)
+    m = hip.RegionMap(rmap, nroi)
+    L = hip.lib()
+    node = torch.from_numpy(node_h).cuda()
+    roi = torch.empty((nroi, nvar, P), dtype=torch.float32, device='cuda')
+    back = torch.empty_like(node)
+    torch.cuda.synchronize()
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())
+
+    def run(fn, src, dst, n):
+        for _ in range(n):
+            hip.check(fn(m.h, nvar, P, vp(src), vp(dst)))
+        m.ctx.sync()
+
+    res = {}
+    for name, fn, src, dst in (('to_avg', L.tvbx_rmap_to_avg, node, roi),
+                               ('from_avg', L.tvbx_rmap_from_avg, roi, back)):
+        run(fn, src, dst, W)
+        t0 = time.perf_counter()
+        run(fn, src, dst, K)
+        res[name] = (time.perf_counter() - t0) / K
+    # algorithmic bytes: every vertex value read (to_avg) or written
+    # (from_avg) once, plus the region values and the index arrays
+    vals = nnode * nvar * P
+    b_to = 4 * vals + 4 * nroi * nvar * P + 4 * nnode * nvar
+    b_from = 4 * vals + 4 * nnode
+    ach = b_to / res['to_avg'] / 1e9
+    out = {
+        'metric': 'region map vertex values/s (to_avg)',
+        'value': vals / res['to_avg'], 'unit': 'vertex*var*instance/s',
+        'n_gpus': 1, 'steps': K, 'warmup': W,
+        'ms_per_step': res['to_avg'] * 1e3, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {'workload': '%s: %d vertices, %d regions, %d variables, '
+                               '%d instances' % (args.workload, nnode, nroi,
+                                                 nvar, P)},
+        'roofline': {'bound': 'hbm', 'achieved': ach, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': ach / HBM_PEAK_GBS,
+                     'traffic': None, 'kernel': 'k_rmap_to_avg',
+                     'bytes_per_call': b_to},
+        'from_avg': {'ms_per_call': res['from_avg'] * 1e3,
+                     'achieved_gbs': b_from / res['from_avg'] / 1e9,
+                     'kernel': 'k_rmap_from_avg'},
+    }
+    roi_h = roi.cpu().numpy()
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import oracle as O
+    t0 = time.perf_counter()
+    ref = O.rmap_to_avg(rmap, node_h, nroi)
+    el = time.perf_counter() - t0
+    out['exact_vs_oracle'] = bool(np.array_equal(roi_h, ref))
+    if not args.no_cpu:
+        out['cpu_baseline'] = {'value': vals / el,
+                               'unit': 'vertex*var*instance/s', 'cores': 1,
+                               'kind': 'port',
+                               'sample': 'the whole workload once (numpy '
+                                         'unbuffered add.at, one thread)'}
+    print(json.dumps(out), flush=True)
 
 
 def bench_batch(args):
@@ -389,6 +471,8 @@ def main():
 
     if args.workload in configs.BATCHES:
         return bench_batch(args)
+    if args.workload in RMAP_WORKLOADS:
+        return bench_rmap(args)
     gw = 0 if args.no_graph else args.graph_window
     cfg = configs.ALL[args.workload]()
     K = args.steps or cfg.n_steps

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TVBX_ABI_VERSION 2
+#define TVBX_ABI_VERSION 3
 
 #define TVBX_OK 0
 #define TVBX_E_ARG (-1)      /* invalid argument (validated before any launch) */
@@ -126,6 +126,12 @@ typedef struct tvbx_sim_desc {
                         * of the 8 XCDs (which take workgroup ids round
                         * robin) works on contiguous instance tiles, keeping
                         * the ring lines that several rows read in one L2 */
+  int weight_groups;   /* 0: one weight per edge, weight[nnz]; 1: weights
+                        * per delay group as well, weight[n_group][nnz] --
+                        * subjects sharing one sparsity pattern with their
+                        * own weights and delays (examples/hcp100.py:22-53
+                        * builds that pattern: sW_data / sD_data of shape
+                        * (nsubj, nnz) over one row / col) */
 } tvbx_sim_desc;
 
 enum { TVBX_P_GLOB = 0, TVBX_P_INST = 1, TVBX_P_NODE = 2, TVBX_P_NI = 3 };
@@ -136,8 +142,9 @@ int tvbx_sim_destroy(tvbx_sim *sim);
 
 /* CSR connectome (network.py:66-78 Connectivity.from_dense: row = target,
  * col = source) with integer delays per group, delays[n_group][nnz]
- * (hackathon.py:44 computes them from lengths / speed / dt). Validates
- * row[N] == nnz, col < N, 0 <= delay <= horizon - 2. */
+ * (hackathon.py:44 computes them from lengths / speed / dt), and weights
+ * weight[nnz], or weight[n_group][nnz] with desc.weight_groups. Validates
+ * row[N] == nnz, col < N, 0 <= delay <= horizon - 2, finite weights. */
 int tvbx_sim_set_connectome(tvbx_sim *sim, const int32_t *row,
                             const int32_t *col, const float *weight,
                             const int32_t *delays);
@@ -192,6 +199,16 @@ int tvbx_sim_fc(tvbx_sim *sim, int p0, int np, float *corr,
                 long long *n_samples);
 /* restart FC accumulation (keeps the simulation state) */
 int tvbx_sim_fc_reset(tvbx_sim *sim);
+/* FC-SC error of the FC accumulated since the last reset, on the device:
+ * err[q] = sum over node pairs i < j of (corr_ij - sc_ij)^2 for instance
+ * p0 + q, corr in FP64 from the co-moments, sc an N x N host matrix (row
+ * major; its strict upper triangle is read).  With tvbx_sim_fc_reset after
+ * each window this is the windowed acceptance statistic of
+ * examples/hackathon.py:64-81 (numpy.corrcoef per window of samples,
+ * ((fc[r, c] - weights[r, c])**2).sum() over triu_indices(N, 1)) without
+ * the sample series or the matrices leaving the device. */
+int tvbx_sim_fc_sc_error(tvbx_sim *sim, const float *sc, int p0, int np,
+                         double *err, long long *n_samples);
 /* per-instance flag: 1 if a state variable became non-finite */
 int tvbx_sim_get_flags(tvbx_sim *sim, uint32_t *flags);
 /* device time of the step kernels since the last reset, measured with
@@ -234,6 +251,28 @@ int tvbx_rng_normals(tvbx_ctx *ctx, unsigned long long seed, float *out,
  * [n][N][P] (host) -> corr [P][N][N] (host), FP64 accumulation. */
 int tvbx_corr(tvbx_ctx *ctx, const float *samples, long long n, int N, int P,
               float *corr);
+
+/* ---- region mapping: surface vertices <-> regions ---------------------------
+ * Reference: RMapToAvg / RMapFromAvg (tvb_hpc/network.py:174-202; known
+ * answers tvb_hpc/tests.py:453-473).  rmap[k] in [0, nroi) is the region of
+ * vertex k.  Arrays are [rows][nvar][n_inst] with the instance index
+ * innermost, so n_inst = 1 is the reference's (rows, nvar) layout.
+ *   to_avg:   roi[r][j][p] = sum of node[k][j][p] over the k with rmap[k] = r
+ *             (a sum, as the reference kernel computes despite its name),
+ *             added in ascending k: float32 results equal the reference's.
+ *   from_avg: node[k][j][p] = roi[rmap[k]][j][p].
+ * Each array pointer may be device memory (used in place; the call is then
+ * asynchronous on the context's stream) or host memory (staged through the
+ * device; the call returns when the result is in place).  An rmap entry out
+ * of range is TVBX_E_ARG at create. */
+typedef struct tvbx_rmap tvbx_rmap;
+int tvbx_rmap_create(tvbx_ctx *ctx, const int *rmap, int nnode, int nroi,
+                     tvbx_rmap **out);
+int tvbx_rmap_destroy(tvbx_rmap *rmap);
+int tvbx_rmap_to_avg(tvbx_rmap *rmap, int nvar, int n_inst, const float *node,
+                     float *roi);
+int tvbx_rmap_from_avg(tvbx_rmap *rmap, int nvar, int n_inst,
+                       const float *roi, float *node);
 
 /* ---- batches: subjects with different connectomes in one launch -----------
  * Reference: the multi-subject harnesses run one compiled kernel per

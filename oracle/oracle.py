@@ -101,6 +101,7 @@ class OracleSpec(ctypes.Structure):
         ('ring_q', ctypes.c_int), ('ring_pre', ctypes.c_int),
         ('ring_scale', ctypes.c_double),
         ('inst_off', ctypes.c_int),
+        ('weight_groups', ctypes.c_int),
     ]
 
 
@@ -202,6 +203,22 @@ def fc_f64(samples):
     return out
 
 
+def rmap_to_avg(rmap, node, nroi):
+    """Region sums (RMapToAvg, tvb_hpc/network.py:174-190): roi zeroed, then
+    roi[rmap[k]] += node[k] for k = 0, 1, ... in order, in node's dtype.
+    node (nnode, nvar[, P]) -> roi (nroi, nvar[, P])."""
+    node = np.asarray(node)
+    roi = np.zeros((nroi, ) + node.shape[1:], node.dtype)
+    # unbuffered: repeated indices accumulate one at a time, in index order
+    np.add.at(roi, np.asarray(rmap).reshape(-1), node)
+    return roi
+
+
+def rmap_from_avg(rmap, roi):
+    """RMapFromAvg (tvb_hpc/network.py:193-202): node[k] = roi[rmap[k]]."""
+    return np.asarray(roi)[np.asarray(rmap).reshape(-1)]
+
+
 def _expand_par(names, defaults, overrides, N, P, dtype, per_node=True):
     """Parameter block [len(names)][N][P] (or [len(names)][P])."""
     shape = (len(names), N, P) if per_node else (len(names), P)
@@ -268,7 +285,13 @@ class OracleSim:
         if self.delay.size and self.delay.max() > H - 2:
             raise ValueError('max delay %d needs a ring of at least %d slots'
                              % (self.delay.max(), self.delay.max() + 2))
-        self.weight = np.ascontiguousarray(weight, dtype)
+        # weight (nnz,) shared, or (n_group, nnz): one row per delay group
+        w = np.asarray(weight)
+        self.weight_groups = int(w.ndim == 2)
+        if self.weight_groups and w.shape != self.delay.shape:
+            raise ValueError('grouped weights must match delays (n_group, '
+                             'nnz)')
+        self.weight = np.ascontiguousarray(w, dtype)
         params = params or {}
         cparams = cparams or {}
         self.par = _expand_par(MODEL_PARAMS[model], MODEL_DEFAULTS[model],
@@ -412,6 +435,7 @@ class OracleSim:
         sp.ring_pre = int(self.ring_pre)
         sp.ring_scale = self.ring_scale
         sp.inst_off = self.inst_off
+        sp.weight_groups = self.weight_groups
         f = lib().oracle_run_f64 if self.dtype == np.float64 else \
             lib().oracle_run_f32
         rc = f(ctypes.byref(sp))

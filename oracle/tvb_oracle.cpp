@@ -86,6 +86,9 @@ typedef struct {
   double ring_scale;
   int inst_off;                   // instance index of the Philox counter is
                                   // inst_off + p (a subject of a batch)
+  int weight_groups;              // 1: weight is T[n_group][nnz], one row
+                                  // per delay group (examples/hcp100.py:
+                                  // per-subject sW_data over one pattern)
 } OracleSpec;
 
 }  // extern "C"
@@ -425,6 +428,9 @@ void run_model(OracleSpec *sp) {
     std::vector<T> cp(n_cpar > 0 ? n_cpar : 1), pv(n_par > 0 ? n_par : 1);
     for (int k = 0; k < n_cpar; ++k) cp[k] = cpar[(size_t)k * P + p];
     const int *dl = sp->delay + (size_t)(p / sp->group_size) * sp->nnz;
+    const T *wg = weight + (sp->weight_groups
+                                ? (size_t)(p / sp->group_size) * sp->nnz
+                                : 0);
     T *xs = state + (size_t)p * S * N;
     T *rg = ring + (size_t)p * H * R * N;
     T *st = sp->order == 1 ? (T *)sp->stale + (size_t)p * 2 * R * N : nullptr;
@@ -455,7 +461,7 @@ void run_model(OracleSpec *sp) {
                         : rg[((size_t)slot * R + kk) * N + sp->col[e]];
             T term = cf_pre<T>(sp->cfun, pre, post, cp.data());
             if (qpre) term = quant<T>(sp->ring_q, term * rscale);
-            acc = std::fma(weight[e], term, acc);
+            acc = std::fma(wg[e], term, acc);
           }
           if (qpre) acc = acc * runscale;
           c[kk] = cf_post<T>(sp->cfun, acc, N, cp.data());

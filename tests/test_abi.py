@@ -32,7 +32,7 @@ def test_library_exports_every_declared_function():
 
 def test_abi_version_and_errors():
     L = hip.lib()
-    assert L.tvbx_abi_version() == 2
+    assert L.tvbx_abi_version() == 3
     assert L.tvbx_last_error(None, 0) == -1
     n = ctypes.c_int(-1)
     assert L.tvbx_device_count(ctypes.byref(n)) == 0

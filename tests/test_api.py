@@ -283,3 +283,11 @@ def test_mpr_workload_starts_in_a_stable_basin():
     ref.init_history('zeros')
     ref.run(20000, seed=5)
     assert np.isfinite(ref.get_state()).all()
+
+
+def test_region_map_rejects_bad_indices_before_the_device():
+    from tvb_hpc.hip import RegionMap
+    with pytest.raises(ValueError):
+        RegionMap(np.array([0, 1, 5]), nroi=3)
+    with pytest.raises(ValueError):
+        RegionMap(np.array([0, -1, 2]), nroi=3)

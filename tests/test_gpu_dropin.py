@@ -129,3 +129,52 @@ def test_hackathon_known_answer_on_device():
     print('device derr_speed=%.1f derr_coupl=%.1f' % (ds, dc))
     assert ds > 350.0
     assert dc < -500.0
+
+
+def test_hackathon_known_answer_fc_sc_on_device():
+    """The same acceptance check with the windowed FC and the FC-SC error
+    computed on the device (FP64 co-moments restarted per 2 s window): only
+    the (windows, P) errors come back; they equal the host statistic of the
+    same samples, and the trend assertions hold."""
+    from tvb_hpc import metric
+    conn, couplings, speeds = workload()
+    N = conn.nnode
+    osc = model.Kuramoto()
+    cfun = coupling.Kuramoto(osc)
+    delays = np.stack([(conn.lnz / s / 1.0).astype(np.uintc)
+                       for s in speeds]).astype(np.int32)
+    csr = CSR(N, conn.row, conn.col, conn.wnz, delays)
+    G = 64
+    P = G * speeds.size
+    a = np.tile(np.resize(couplings, G), speeds.size).astype(np.float32)
+    H = int(delays.max()) + 3 + 4000
+    kw = dict(horizon=H, params={'omega': 2 * np.pi * 10 / 1e3, 'c:a': a},
+              kinds={'m:omega': 'glob'}, group_size=G, tavg_period=10,
+              order='reference', waves=8)
+    th0 = np.zeros((1, N, P), np.float32)
+    items = list(itertools.product(range(speeds.size), range(couplings.size)))
+    th_real = initial_theta(N, len(items))
+    for j, (si, ci) in enumerate(items):
+        th0[0, :, si * G + ci] = th_real[:, j]
+    cols = [si * G + ci for si, ci in items]
+    sim = Simulation(osc, cfun, scheme.EulerStep(1.0), csr, P, fc=True,
+                     sample_slots=64, **kw)
+    sim.set_state(th0)
+    sim.init_history('zeros')
+    err = metric.windowed_fc_sc_error(sim, conn.weights, 2, 2000)
+    assert err.shape == (2, P)
+    # the same run keeping its samples: host statistic per window
+    sim2 = Simulation(osc, cfun, scheme.EulerStep(1.0), csr, P,
+                      sample_slots=400, **kw)
+    sim2.set_state(th0)
+    sim2.init_history('zeros')
+    sim2.run(4000)
+    smp, _ = sim2.samples()
+    for w in range(2):
+        fc64 = O.fc_f64(smp[w * 200:(w + 1) * 200, :, cols].astype(np.float64))
+        ref = metric.fc_sc_error(fc64, conn.weights.astype(np.float64))
+        np.testing.assert_allclose(err[w, cols], ref, rtol=1e-9, atol=1e-9)
+    ds, dc = metric.sweep_trends(err[-1, cols], speeds.size, couplings.size)
+    print('device FC-SC derr_speed=%.1f derr_coupl=%.1f' % (ds, dc))
+    assert ds > 350.0
+    assert dc < -500.0

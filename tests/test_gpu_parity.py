@@ -18,6 +18,7 @@ from harness import (CASES, CFUNS, MODELS, oracle_for, scaled_err, small_csr,
                      state_err)
 from tvb_hpc import scheme as SC
 from tvb_hpc.hip import Simulation, TvbxError
+from tvb_hpc.network import CSR
 
 pytestmark = pytest.mark.gpu
 
@@ -493,3 +494,44 @@ def test_reference_rww_singularity_is_flagged():
                        em=False)
     sim2.run(1000)
     assert np.isfinite(sim2.get_state()).all() and not sim2.flags().any()
+
+
+def _group_sparse_csr(n_subj=3, N=18, dt=0.1, seed=21):
+    """subjects sharing the pattern of edges present in all of them, with
+    their own weights and lengths (examples/hcp100.py:22-53)"""
+    rng = np.random.default_rng(seed)
+    base = rng.random((N, N)) < 0.45
+    W = np.stack([np.where(base, rng.uniform(0.1, 1.0, (N, N)), 0.0)
+                  for _ in range(n_subj)])
+    L = np.stack([rng.uniform(1.0, 40.0, (N, N)) for _ in range(n_subj)])
+    return CSR.group_sparse(W, L, [1.0, 2.0, 3.5][:n_subj], dt)
+
+
+@pytest.mark.parametrize('launch', [dict(waves=4, node_block=5),
+                                    'blocked'])
+def test_grouped_weights_match_oracle(launch):
+    """One weight row and one delay row per group of 64 instances (subjects
+    on a shared pattern), per-step and temporally blocked launches."""
+    dt = 0.1
+    csr = _group_sparse_csr(dt=dt)
+    assert csr.weight_groups and csr.weight.shape == csr.delays.shape
+    N, P, gs = csr.nnode, 192, 64
+    if launch == 'blocked':
+        L = min(int(csr.delays.min()) + 1, 4)
+        assert L >= 2
+        launch = dict(waves=2, node_block=2, steps_per_launch=L)
+    m = MODELS['Generic2dOscillator']()
+    c = CFUNS['Sigmoidal'](m)
+    G = np.linspace(0, 1, P).astype(np.float32)
+    sim = Simulation(m, c, SC.EulerStep(dt), csr, P, params={'c:a': G},
+                     group_size=gs, **launch)
+    x0 = m.initial_state(N, P, np.random.default_rng(4))
+    sim.set_state(x0)
+    sim.init_history('zeros')
+    ref = oracle_for('Generic2dOscillator', 'Sigmoidal', csr, P, sim.H, dt,
+                     'euler', cparams={'a': G}, group_size=gs)
+    ref.set_state(x0)
+    ref.init_history('zeros')
+    sim.run(600)
+    ref.run(600)
+    assert scaled_err(sim.get_state(), ref.get_state()) < TOL

@@ -439,3 +439,104 @@ def test_u8_history_error_bound():
     assert d[4].max() > 10 * np.spacing(np.abs(out[0][4])).max()
     assert np.all(d[[0, 1, 2, 3, 5]] <= 1e-9)
     assert d[4].max() > 0
+
+
+# ------------------------------------------------------------ region maps
+
+def test_rmap_oracle_reference_known_answers():
+    """tvb_hpc/tests.py:453-473: 16 vertices in 4 regions of 4; to_avg
+    gives the row sums of node reshaped (4, 4), from_avg of roi = [0..3]
+    gives rmap itself."""
+    i = np.r_[:16].reshape((-1, 1))
+    rmap = i // 4
+    node = i.astype('f')
+    roi = O.rmap_to_avg(rmap, node, 4)
+    np.testing.assert_allclose(roi[:, 0], node.reshape((4, 4)).sum(axis=1))
+    back = O.rmap_from_avg(rmap, np.r_[:4].reshape((4, 1)).astype('f'))
+    np.testing.assert_allclose(rmap, back)
+
+
+def test_rmap_oracle_order_and_invariants():
+    """to_avg adds in ascending vertex order: equal, bit for bit, to the
+    reference kernel's loop written out; the total over regions equals the
+    total over vertices for integer-valued data; an empty region is 0;
+    from_avg(to_avg) of a region-constant field scales by region size."""
+    rng = np.random.default_rng(3)
+    nnode, nroi, nvar, P = 300, 7, 2, 5
+    rmap = rng.integers(0, nroi - 1, size=nnode)      # region nroi-1 empty
+    node = rng.standard_normal((nnode, nvar, P)).astype(np.float32)
+    roi = O.rmap_to_avg(rmap, node, nroi)
+    loop = np.zeros((nroi, nvar, P), np.float32)
+    for j in range(nvar):
+        for k in range(nnode):
+            loop[rmap[k], j] = loop[rmap[k], j] + node[k, j]
+    assert np.array_equal(roi, loop)
+    assert not roi[nroi - 1].any()
+    ints = rng.integers(-50, 50, size=(nnode, nvar, P)).astype(np.float32)
+    np.testing.assert_array_equal(O.rmap_to_avg(rmap, ints, nroi).sum(0),
+                                  ints.sum(0))
+    const = rng.standard_normal((nroi, 1, 1)).astype(np.float32)
+    field = O.rmap_from_avg(rmap, const)
+    size = np.bincount(rmap, minlength=nroi).astype(np.float64)
+    np.testing.assert_allclose(O.rmap_to_avg(rmap, field.astype(np.float64),
+                                             nroi)[:, 0, 0],
+                               size * const[:, 0, 0], rtol=1e-12)
+
+
+# ------------------------------------------------------ grouped weights
+
+def test_oracle_grouped_weights_equal_separate_runs():
+    """An instance of delay group g with weights per group steps exactly as
+    the same instance in a run whose only weights and delays are group g's
+    (the grouped layout only selects rows)."""
+    from tvb_hpc.network import CSR
+    rng = np.random.default_rng(8)
+    csr = small_csr(N=14, speeds=(1.0, 3.0))
+    w2 = np.stack([csr.weight, rng.uniform(0.1, 1.0, csr.nnz)]).astype(
+        np.float32)
+    P, gs = 4, 2
+    G = np.linspace(0.1, 0.9, P)
+    x0 = MODELS['Generic2dOscillator']().initial_state(
+        14, P, np.random.default_rng(2))
+    kw = dict(dt=0.1, scheme='euler', cparams={'a': G})
+    grp = O.OracleSim('Generic2dOscillator', 'Sigmoidal', 14, csr.row,
+                      csr.col, w2, csr.delays, P, csr.max_delay + 2,
+                      group_size=gs, **kw)
+    grp.set_state(x0)
+    grp.init_history('zeros')
+    grp.run(300)
+    for g in range(2):
+        one = O.OracleSim('Generic2dOscillator', 'Sigmoidal', 14, csr.row,
+                          csr.col, w2[g], csr.delays[g], P,
+                          csr.max_delay + 2, **kw)
+        one.set_state(x0)
+        one.init_history('zeros')
+        one.run(300)
+        sl = slice(g * gs, (g + 1) * gs)
+        np.testing.assert_array_equal(grp.get_state()[..., sl],
+                                      one.get_state()[..., sl])
+    # and the groups do differ
+    assert not np.array_equal(grp.get_state()[..., :gs],
+                              grp.get_state()[..., gs:])
+
+
+def test_group_sparse_csr_builder():
+    """CSR.group_sparse keeps the edges present in every subject (off the
+    diagonal), in row-major order, with each subject's own weights and
+    truncated delays."""
+    from tvb_hpc.network import CSR
+    rng = np.random.default_rng(5)
+    S, N = 4, 9
+    W = rng.uniform(0.1, 1.0, (S, N, N)) * (rng.random((S, N, N)) < 0.8)
+    L = rng.uniform(1.0, 50.0, (S, N, N))
+    csr = CSR.group_sparse(W, L, [1.0, 2.0, 4.0, 8.0], 0.1)
+    mask = (W != 0).all(axis=0) & ~np.eye(N, dtype=bool)
+    assert csr.nnz == mask.sum()
+    assert csr.weight.shape == (S, csr.nnz) == csr.delays.shape
+    for s, v in enumerate([1.0, 2.0, 4.0, 8.0]):
+        dense = csr.dense_weights(group=s)
+        np.testing.assert_allclose(dense, np.where(mask, W[s], 0.0),
+                                   rtol=1e-6)
+        rows = np.repeat(np.arange(N), np.diff(csr.row))
+        np.testing.assert_array_equal(
+            csr.delays[s], np.floor(L[s][rows, csr.col] / v / 0.1))

@@ -33,7 +33,8 @@ tvbx_adjoint(const tvbx_adj_args A) {
   const int group = __builtin_amdgcn_readfirstlane(
       (blockIdx.x * 64 * TVBX_IPL) / a.group_size);
   const unsigned *__restrict__ crow = A.crow + (size_t)group * a.nnz;
-  const float *__restrict__ cw = A.cw;
+  const float *__restrict__ cw =
+      A.cw + (a.weight_groups ? (size_t)group * a.nnz : 0);
   const int *__restrict__ ccol = A.ccol;
   const float *__restrict__ mu_r = A.mu;
   const int i_begin = blockIdx.y * a.node_block;

@@ -276,8 +276,16 @@ __device__ __forceinline__ void tvbx_load_row_of(
 
 __device__ __forceinline__ void tvbx_load_row(const tvbx_step_args &a,
                                               const unsigned *__restrict__ cd,
+                                              const float *__restrict__ wt,
                                               int i, tvbx_row &r) {
-  tvbx_load_row_of(a.row, cd, a.weight, i, r);
+  tvbx_load_row_of(a.row, cd, wt, i, r);
+}
+
+// the weights of an instance tile's delay group (one row of nnz per group
+// when weights are grouped, else the shared row)
+__device__ __forceinline__ const float *tvbx_group_weights(
+    const tvbx_step_args &a, int group) {
+  return a.weight + (a.weight_groups ? (size_t)group * a.nnz : 0);
 }
 
 __device__ __forceinline__ unsigned tvbx_lane_u(unsigned v, int l) {
@@ -425,12 +433,12 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
 // recording for the adjoint.
 __device__ __forceinline__ void tvbx_coupling(
     const tvbx_step_args &a, const unsigned *__restrict__ cd,
-    const tvbx_row &row, const tvbx_rt *__restrict__ ring,
+    const float *__restrict__ wt, const tvbx_row &row,
+    const tvbx_rt *__restrict__ ring,
     const tvbx_rt *__restrict__ st_cur, int i, int t, int pb, long long n,
     const tvbx_lane_params (&lp)[TVBX_IPL], float (&c)[TVBX_IPL][TVBX_C]) {
   const size_t NP = (size_t)a.n_node * a.pitch;
   const size_t ip = (size_t)i * a.pitch + pb;
-  const float *__restrict__ wt = a.weight;
   const unsigned lane_off = (unsigned)pb * (unsigned)sizeof(tvbx_rt);
   const int deg = row.e1 - row.e0;
   const int n0 = min(deg, 64);
@@ -691,6 +699,7 @@ tvbx_step(const tvbx_step_args a) {
   const int group = __builtin_amdgcn_readfirstlane(
       (bx * 64 * TVBX_IPL) / a.group_size);
   const unsigned *__restrict__ cd = a.coldelay + (size_t)group * a.nnz;
+  const float *__restrict__ wt = tvbx_group_weights(a, group);
   const tvbx_rt *__restrict__ ring = (const tvbx_rt *)a.ring;
   tvbx_rt *ring_w = (tvbx_rt *)a.ring;
   const int i_begin = by * a.node_block;
@@ -730,8 +739,8 @@ tvbx_step(const tvbx_step_args a) {
       if (a.tavg_period > 0) tvbx_monitor_obs(a, i, pb, lp, x, y_mon);
 
       tvbx_row rw;
-      tvbx_load_row(a, cd, i, rw);
-      tvbx_coupling(a, cd, rw, ring, st_cur, i, t, pb, n, lp, c);
+      tvbx_load_row(a, cd, wt, i, rw);
+      tvbx_coupling(a, cd, wt, rw, ring, st_cur, i, t, pb, n, lp, c);
 
       float yo[TVBX_R][TVBX_IPL];
       tvbx_advance(a, i, pb, n, k, lp, x, c, yo, bad);
@@ -797,6 +806,7 @@ __device__ __forceinline__ void tvbx_blocked_run(const tvbx_step_args &a,
   const int group = __builtin_amdgcn_readfirstlane(
       (bx * 64 * TVBX_IPL) / a.group_size);
   const unsigned *__restrict__ cd = a.coldelay + (size_t)group * a.nnz;
+  const float *__restrict__ wt = tvbx_group_weights(a, group);
   const tvbx_rt *__restrict__ ring = (const tvbx_rt *)a.ring;
   tvbx_rt *ring_w = (tvbx_rt *)a.ring;
   const size_t ip = (size_t)i * a.pitch + pb;
@@ -821,7 +831,7 @@ __device__ __forceinline__ void tvbx_blocked_run(const tvbx_step_args &a,
   if (a.tavg_period > 0) tvbx_ldf(a.tavg + ip, acc_t);
   // the node's row serves every step of the launch
   tvbx_row rw;
-  tvbx_load_row(a, cd, i, rw);
+  tvbx_load_row(a, cd, wt, i, rw);
 
   for (int k = 0; k < a.n_steps; ++k) {
     const long long n = n0 + k;
@@ -830,7 +840,7 @@ __device__ __forceinline__ void tvbx_blocked_run(const tvbx_step_args &a,
     float y_mon[TVBX_IPL];
     if (a.tavg_period > 0) tvbx_monitor_obs(a, i, pb, lp, x, y_mon);
     float c[TVBX_IPL][TVBX_C];
-    tvbx_coupling(a, cd, rw, ring, nullptr, i, t, pb, n, lp, c);
+    tvbx_coupling(a, cd, wt, rw, ring, nullptr, i, t, pb, n, lp, c);
     float yo[TVBX_R][TVBX_IPL];
     tvbx_advance(a, i, pb, n, k, lp, x, c, yo, bad);
 #pragma unroll

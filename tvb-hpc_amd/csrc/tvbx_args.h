@@ -103,6 +103,8 @@ typedef struct tvbx_step_args {
   /* 1: the ring is smaller than 4 GiB, so gathers address it with 32-bit
    * byte offsets from one buffer descriptor */
   int ring32;
+  /* 1: weight holds one row of nnz weights per delay group */
+  int weight_groups;
 } tvbx_step_args;
 
 /* Argument block of the batched step kernel: per subject (one connectome

@@ -573,6 +573,36 @@ __global__ void k_fc_corr(const double *__restrict__ m2, int N, int pitch,
   }
 }
 
+// FC-SC error of row i: part[i][p] = sum over j > i of (corr_ij - sc_ij)^2,
+// corr in FP64 from the co-moments (hackathon.py:74-80 compares
+// numpy.corrcoef, FP64, with the weights over triu_indices(N, 1)).
+// sc_row[j] holds sc_ij of this row as FP64.  grid (pitch / 64, N), block 64.
+__global__ void k_fc_sc_row(const double *__restrict__ m2, int N, int pitch,
+                            const double *__restrict__ sc,
+                            double *__restrict__ part) {
+  const int p = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y;
+  const double mii = m2[tri_index(i, i, N) * pitch + p];
+  const double *__restrict__ sci = sc + (size_t)i * N;
+  double acc = 0.0;
+  for (int j = i + 1; j < N; ++j) {
+    const double mjj = m2[tri_index(j, j, N) * pitch + p];
+    const double c = m2[tri_index(i, j, N) * pitch + p] / sqrt(mii * mjj);
+    const double d = c - sci[j];
+    acc = fma(d, d, acc);
+  }
+  part[(size_t)i * pitch + p] = acc;
+}
+
+// err[p] = sum over rows i (ascending) of part[i][p].  grid pitch / 64.
+__global__ void k_fc_sc_sum(const double *__restrict__ part, int N, int pitch,
+                            double *__restrict__ err) {
+  const int p = blockIdx.x * 64 + threadIdx.x;
+  double acc = 0.0;
+  for (int i = 0; i < N; ++i) acc += part[(size_t)i * pitch + p];
+  err[p] = acc;
+}
+
 // RNG.fill: counters (i, i >> 32, 0, 0), 4 normals per counter
 __global__ void k_normals(float *out, size_t n, unsigned klo, unsigned khi) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -698,6 +728,7 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
     a.xcd_map = d.xcd_map && (gx * gy) % 8 == 0;
   }
   a.ring32 = (double)d.horizon * d.n_ring * s->NP * s->esz < 4294967296.0;
+  a.weight_groups = d.weight_groups;
   a.ring_unscale = d.ring_type == 0   ? 1.0f
                    : d.ring_type == 3 ? 1.0f / (255.0f * d.ring_scale)
                                       : 1.0f / d.ring_scale;
@@ -727,6 +758,8 @@ extern "C" int tvbx_sim_create(tvbx_ctx *c, tvbx_module *m,
   CHECK(d.waves >= 1 && d.waves <= 16, "waves must be in [1, 16]");
   CHECK(d.node_block >= 1, "node_block must be positive");
   CHECK(d.steps_per_launch >= 1, "steps_per_launch must be positive");
+  CHECK(d.weight_groups == 0 || d.weight_groups == 1,
+        "weight_groups must be 0 or 1");
   // several steps per launch: either one workgroup holds all nodes of its
   // instance tile (barriers between steps), or temporal blocking with one
   // node per wave, which the delays must allow (checked with the connectome)
@@ -768,7 +801,8 @@ extern "C" int tvbx_sim_create(tvbx_ctx *c, tvbx_module *m,
   if (rc == TVBX_OK) rc = dalloc(s->buf, (size_t)(bytes));
   ALLOC(row, (N + 1) * 4);
   ALLOC(coldelay, (size_t)d.n_group * std::max(d.nnz, 1) * 4);
-  ALLOC(weight, (size_t)std::max(d.nnz, 1) * 4);
+  ALLOC(weight, (size_t)(d.weight_groups ? d.n_group : 1) *
+                    std::max(d.nnz, 1) * 4);
   ALLOC(state, (size_t)d.n_svar * NP * 4);
   ALLOC(ring, (size_t)d.horizon * d.n_ring * NP * s->esz);
   ALLOC(p_inst, (size_t)std::max(d.n_pinst, 1) * pitch * 4);
@@ -900,15 +934,16 @@ extern "C" int tvbx_sim_set_connectome(tvbx_sim *s, const int32_t *row,
             dl, e, g, d.steps_per_launch, d.horizon);
       cd[(size_t)g * d.nnz + e] = ((unsigned)dl << 16) | (unsigned)col[e];
     }
-  for (int e = 0; e < d.nnz; ++e)
-    CHECK(std::isfinite(weight[e]), "weight[%d] is not finite", e);
+  const size_t nw = (size_t)(d.weight_groups ? d.n_group : 1) * d.nnz;
+  for (size_t e = 0; e < nw; ++e)
+    CHECK(std::isfinite(weight[e]), "weight[%zu] is not finite", e);
   HIPCHK(hipSetDevice(s->ctx->device));
   HIPCHK(hipMemcpyAsync(s->row.p, row, (size_t)(d.n_node + 1) * 4,
                         hipMemcpyHostToDevice, s->ctx->stream));
   if (d.nnz) {
     HIPCHK(hipMemcpyAsync(s->coldelay.p, cd.data(), cd.size() * 4,
                           hipMemcpyHostToDevice, s->ctx->stream));
-    HIPCHK(hipMemcpyAsync(s->weight.p, weight, (size_t)d.nnz * 4,
+    HIPCHK(hipMemcpyAsync(s->weight.p, weight, nw * 4,
                           hipMemcpyHostToDevice, s->ctx->stream));
   }
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
@@ -916,7 +951,7 @@ extern "C" int tvbx_sim_set_connectome(tvbx_sim *s, const int32_t *row,
   // host copy for the adjoint's transpose (rebuilt on demand)
   s->h_row.assign(row, row + d.n_node + 1);
   s->h_col.assign(col, col + d.nnz);
-  s->h_w.assign(weight, weight + d.nnz);
+  s->h_w.assign(weight, weight + nw);
   s->h_delays.assign(delays, delays + (size_t)d.n_group * d.nnz);
   dfree(s->ccol);
   return TVBX_OK;
@@ -1443,6 +1478,50 @@ extern "C" int tvbx_sim_fc(tvbx_sim *s, int p0, int np, float *corr,
   return TVBX_OK;
 }
 
+extern "C" int tvbx_sim_fc_sc_error(tvbx_sim *s, const float *sc, int p0,
+                                    int np, double *err, long long *n_samples) {
+  CHECK(s && sc && err, "null argument");
+  const tvbx_sim_desc &d = s->d;
+  CHECK(d.fc, "sim created without fc");
+  CHECK(p0 >= 0 && np >= 0 && p0 + np <= d.n_inst, "instance range");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  int rc = fc_flush(s);
+  if (rc != TVBX_OK) return rc;
+  if (n_samples) *n_samples = s->samples_used;
+  if (np == 0) return TVBX_OK;
+  CHECK(s->samples_used >= 2, "FC needs at least two samples");
+  const int N = d.n_node;
+  std::vector<double> sc64((size_t)N * N);
+  for (size_t k = 0; k < sc64.size(); ++k) sc64[k] = (double)sc[k];
+  DevBuf dsc, part, derr;
+  rc = dalloc(dsc, sc64.size() * 8);
+  if (rc == TVBX_OK) rc = dalloc(part, (size_t)N * s->pitch * 8);
+  if (rc == TVBX_OK) rc = dalloc(derr, (size_t)s->pitch * 8);
+  hipError_t e = hipSuccess;
+  if (rc == TVBX_OK)
+    e = hipMemcpyAsync(dsc.p, sc64.data(), sc64.size() * 8,
+                       hipMemcpyHostToDevice, s->ctx->stream);
+  if (rc == TVBX_OK && e == hipSuccess) {
+    hipLaunchKernelGGL(k_fc_sc_row, dim3(s->pitch / 64, N), dim3(64), 0,
+                       s->ctx->stream, (const double *)s->fc_m2.p, N, s->pitch,
+                       (const double *)dsc.p, (double *)part.p);
+    hipLaunchKernelGGL(k_fc_sc_sum, dim3(s->pitch / 64), dim3(64), 0,
+                       s->ctx->stream, (const double *)part.p, N, s->pitch,
+                       (double *)derr.p);
+    e = hipGetLastError();
+  }
+  if (rc == TVBX_OK && e == hipSuccess)
+    e = hipMemcpyAsync(err, (const double *)derr.p + p0, (size_t)np * 8,
+                       hipMemcpyDeviceToHost, s->ctx->stream);
+  if (rc == TVBX_OK && e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+  if (rc == TVBX_OK && e != hipSuccess)
+    rc = fail(TVBX_E_HIP, "tvbx_sim_fc_sc_error: %s", hipGetErrorString(e));
+  dfree(dsc);
+  dfree(part);
+  dfree(derr);
+  return rc;
+}
+
 extern "C" int tvbx_sim_fc_reset(tvbx_sim *s) {
   CHECK(s && s->d.fc, "sim created without fc");
   HIPCHK(hipSetDevice(s->ctx->device));
@@ -1539,6 +1618,196 @@ extern "C" int tvbx_corr(tvbx_ctx *c, const float *samples, long long n,
   return rc;
 }
 
+// ------------------------------------------------------------ region maps
+// Surface vertices <-> regions (the reference's RMapToAvg / RMapFromAvg,
+// tvb_hpc/network.py:174-202).  Arrays are [rows][nvar][n_inst] with the
+// instance innermost; one lane per instance.
+
+struct tvbx_rmap {
+  tvbx_ctx *ctx = nullptr;
+  int nnode = 0, nroi = 0;
+  DevBuf rmap;     // [nnode] region of each vertex
+  DevBuf rptr;     // [nroi + 1] CSR of the vertices of each region
+  DevBuf members;  // [nnode] vertices by region, ascending within a region
+};
+
+// roi[r][j][p] = sum of node[k][j][p] over the vertices k of region r, added
+// in ascending k from 0 (the reference kernel's order: one pass over k
+// adding into roi[rmap[k]]), so float32 results equal its sequential ones.
+// The loads of RMAP_UNROLL vertices are in flight together and added in
+// order.  grid (nroi * nvar, ceil(n_inst / 64)), block 64.
+#define RMAP_UNROLL 16
+__global__ void __launch_bounds__(64)
+k_rmap_to_avg(const int *__restrict__ rptr, const int *__restrict__ members,
+              int nvar, int n_inst, const float *__restrict__ node,
+              float *__restrict__ roi) {
+  const int rj = blockIdx.x;
+  const int r = rj / nvar, j = rj - r * nvar;
+  const int p = blockIdx.y * 64 + threadIdx.x;
+  const bool on = p < n_inst;
+  const int m0 = rptr[r], m1 = rptr[r + 1];
+  const size_t row = (size_t)nvar * n_inst;
+  const float *__restrict__ src = node + (size_t)j * n_inst + (on ? p : 0);
+  float acc = 0.0f;
+  int m = m0;
+  for (; m + RMAP_UNROLL <= m1; m += RMAP_UNROLL) {
+    float v[RMAP_UNROLL];
+#pragma unroll
+    for (int u = 0; u < RMAP_UNROLL; ++u)
+      v[u] = src[(size_t)members[m + u] * row];
+#pragma unroll
+    for (int u = 0; u < RMAP_UNROLL; ++u) acc += v[u];
+  }
+  for (; m < m1; ++m) acc += src[(size_t)members[m] * row];
+  if (on) roi[(size_t)rj * n_inst + p] = acc;
+}
+
+// node[k][j][p] = roi[rmap[k]][j][p]: one wave per (k, j) row at a time,
+// rows strided over the grid.  grid (<= 16384, ceil(n_inst / 64)), block
+// 256.
+__global__ void __launch_bounds__(256)
+k_rmap_from_avg(const int *__restrict__ rmap, int nvar, int n_inst,
+                long long rows, const float *__restrict__ roi,
+                float *__restrict__ node) {
+  const int p = blockIdx.y * 64 + (threadIdx.x & 63);
+  const long long w0 = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long ws = (long long)gridDim.x * 4;
+  if (p >= n_inst) return;
+  for (long long kj = w0; kj < rows; kj += ws) {
+    const long long k = kj / nvar;
+    const int j = (int)(kj - k * nvar);
+    node[(size_t)kj * n_inst + p] =
+        roi[((size_t)rmap[k] * nvar + j) * n_inst + p];
+  }
+}
+
+// true when p is device memory that kernels can read and write directly
+static bool on_device(const void *p) {
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice;
+}
+
+extern "C" int tvbx_rmap_create(tvbx_ctx *c, const int *rmap, int nnode,
+                                int nroi, tvbx_rmap **out) {
+  CHECK(c && rmap && out, "null argument");
+  CHECK(nnode >= 1 && nroi >= 1, "need nnode, nroi >= 1");
+  std::vector<int> cnt((size_t)nroi + 1, 0), mem((size_t)nnode);
+  for (int k = 0; k < nnode; ++k) {
+    CHECK(rmap[k] >= 0 && rmap[k] < nroi,
+          "rmap[%d] = %d is not a region index in [0, %d)", k, rmap[k], nroi);
+    ++cnt[(size_t)rmap[k] + 1];
+  }
+  for (int r = 0; r < nroi; ++r) cnt[(size_t)r + 1] += cnt[r];
+  {
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int k = 0; k < nnode; ++k) mem[(size_t)fill[rmap[k]]++] = k;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  tvbx_rmap *r = new tvbx_rmap;
+  r->ctx = c;
+  r->nnode = nnode;
+  r->nroi = nroi;
+  int rc = dalloc(r->rmap, (size_t)nnode * 4);
+  if (rc == TVBX_OK) rc = dalloc(r->rptr, ((size_t)nroi + 1) * 4);
+  if (rc == TVBX_OK) rc = dalloc(r->members, (size_t)nnode * 4);
+  hipError_t e = hipSuccess;
+  if (rc == TVBX_OK)
+    e = hipMemcpy(r->rmap.p, rmap, (size_t)nnode * 4, hipMemcpyHostToDevice);
+  if (rc == TVBX_OK && e == hipSuccess)
+    e = hipMemcpy(r->rptr.p, cnt.data(), cnt.size() * 4,
+                  hipMemcpyHostToDevice);
+  if (rc == TVBX_OK && e == hipSuccess)
+    e = hipMemcpy(r->members.p, mem.data(), mem.size() * 4,
+                  hipMemcpyHostToDevice);
+  if (rc == TVBX_OK && e != hipSuccess)
+    rc = fail(TVBX_E_HIP, "tvbx_rmap_create: %s", hipGetErrorString(e));
+  if (rc != TVBX_OK) {
+    dfree(r->rmap);
+    dfree(r->rptr);
+    dfree(r->members);
+    delete r;
+    return rc;
+  }
+  *out = r;
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_rmap_destroy(tvbx_rmap *r) {
+  if (!r) return TVBX_OK;
+  HIPCHK(hipSetDevice(r->ctx->device));
+  dfree(r->rmap);
+  dfree(r->rptr);
+  dfree(r->members);
+  delete r;
+  return TVBX_OK;
+}
+
+// Run one mapping from src (rows_in x nvar x n_inst) to dst (rows_out x ...)
+// with either pointer in host or device memory; host arrays are staged
+// through device buffers and the call then waits for the result.
+static int rmap_run(tvbx_rmap *r, bool to_avg, int nvar, int n_inst,
+                    const float *src, float *dst) {
+  CHECK(r && src && dst, "null argument");
+  CHECK(nvar >= 1 && n_inst >= 1, "need nvar, n_inst >= 1");
+  CHECK((n_inst + 63) / 64 <= 65535, "n_inst too large");
+  const size_t rows_in = to_avg ? r->nnode : r->nroi;
+  const size_t rows_out = to_avg ? r->nroi : r->nnode;
+  CHECK(rows_out * nvar <= 0x7fffffffu, "too many rows");
+  const size_t bin = rows_in * nvar * n_inst * 4;
+  const size_t bout = rows_out * nvar * n_inst * 4;
+  hipStream_t st = r->ctx->stream;
+  HIPCHK(hipSetDevice(r->ctx->device));
+  const bool dsrc = on_device(src), ddst = on_device(dst);
+  DevBuf tin, tout;
+  int rc = TVBX_OK;
+  if (!dsrc) rc = dalloc(tin, bin);
+  if (rc == TVBX_OK && !ddst) rc = dalloc(tout, bout);
+  const float *s = dsrc ? src : (const float *)tin.p;
+  float *d = ddst ? dst : (float *)tout.p;
+  hipError_t e = hipSuccess;
+  if (rc == TVBX_OK && !dsrc)
+    e = hipMemcpyAsync(tin.p, src, bin, hipMemcpyHostToDevice, st);
+  if (rc == TVBX_OK && e == hipSuccess) {
+    const unsigned tiles = (unsigned)((n_inst + 63) / 64);
+    const long long rows = (long long)rows_out * nvar;
+    if (to_avg)
+      hipLaunchKernelGGL(k_rmap_to_avg, dim3((unsigned)rows, tiles), dim3(64),
+                         0, st, (const int *)r->rptr.p,
+                         (const int *)r->members.p, nvar, n_inst, s, d);
+    else
+      hipLaunchKernelGGL(k_rmap_from_avg,
+                         dim3((unsigned)std::min<long long>(
+                                  (rows + 3) / 4, 16384),
+                              tiles),
+                         dim3(256), 0, st, (const int *)r->rmap.p, nvar,
+                         n_inst, rows, s, d);
+    e = hipGetLastError();
+  }
+  if (rc == TVBX_OK && e == hipSuccess && !ddst)
+    e = hipMemcpyAsync(dst, tout.p, bout, hipMemcpyDeviceToHost, st);
+  if (rc == TVBX_OK && e == hipSuccess && (!dsrc || !ddst))
+    e = hipStreamSynchronize(st);
+  if (rc == TVBX_OK && e != hipSuccess)
+    rc = fail(TVBX_E_HIP, "region map: %s", hipGetErrorString(e));
+  dfree(tin);
+  dfree(tout);
+  return rc;
+}
+
+extern "C" int tvbx_rmap_to_avg(tvbx_rmap *r, int nvar, int n_inst,
+                                const float *node, float *roi) {
+  return rmap_run(r, true, nvar, n_inst, node, roi);
+}
+
+extern "C" int tvbx_rmap_from_avg(tvbx_rmap *r, int nvar, int n_inst,
+                                  const float *roi, float *node) {
+  return rmap_run(r, false, nvar, n_inst, roi, node);
+}
+
 // ------------------------------------------------------------------ adjoint
 
 // sum_j src[g][j][p] -> dst[g][p] (sequential over j: deterministic)
@@ -1581,8 +1850,12 @@ static int build_csc(tvbx_sim *s) {
       tgt[k] = i;
     }
   std::vector<unsigned> crow((size_t)d.n_group * std::max(nnz, 1), 0u);
-  std::vector<float> cw(std::max(nnz, 1), 0.0f);
-  for (int k = 0; k < nnz; ++k) cw[k] = s->h_w[perm[k]];
+  // CSC weights, one row per delay group with grouped weights
+  const int ngw = d.weight_groups ? d.n_group : 1;
+  std::vector<float> cw((size_t)ngw * std::max(nnz, 1), 0.0f);
+  for (int g = 0; g < ngw; ++g)
+    for (int k = 0; k < nnz; ++k)
+      cw[(size_t)g * nnz + k] = s->h_w[(size_t)g * nnz + perm[k]];
   for (int g = 0; g < d.n_group; ++g)
     for (int k = 0; k < nnz; ++k) {
       const int dl = s->h_delays[(size_t)g * nnz + perm[k]];

@@ -72,7 +72,8 @@ class SweepConfig:
         csr = self.csr
         gs = self.group_size
         if gs is not None and n_inst <= gs:
-            csr = CSR(csr.nnode, csr.row, csr.col, csr.weight,
+            csr = CSR(csr.nnode, csr.row, csr.col,
+                      csr.weight[:1] if csr.weight_groups else csr.weight,
                       csr.delays[:1])
             gs = None
         return SweepConfig(self.name, self.model, self.cfun, self.scheme,
@@ -277,6 +278,30 @@ def mpr84(n_inst=4096, n_steps=20000, G=None, tavg_period=10):
         description='MPR N=84, P=%d, sigma 0.01' % n_inst)
 
 
+def hcp100_sparse(n_subj=100, per_subj=64, n_steps=10000, speed=2.0):
+    """Subjects on one sparsity pattern (the group-sparse layout of
+    examples/hcp100.py:22-53): G2DO, linear coupling, Euler, dt 0.1 ms, N=84;
+    each subject has its own weights and lengths over the edges present in
+    every subject (15 % of node pairs, as the reference reports for its 100
+    subjects), and is one delay group of `per_subj` instances sweeping the
+    coupling."""
+    N, dt = 84, 0.1
+    W0, L0 = synth_connectome(N, 0.15, seed=6)
+    rng = np.random.default_rng(60)
+    W = np.stack([W0 * rng.uniform(0.7, 1.3, (N, N)) for _ in range(n_subj)])
+    L = np.stack([np.where(L0 > 0, L0 * rng.uniform(0.9, 1.1, (N, N)), 0.0)
+                  for _ in range(n_subj)])
+    csr = CSR.group_sparse(W, L, speed, dt)
+    G = np.linspace(0.0, 0.5, per_subj)
+    P = n_subj * per_subj
+    osc = M.Generic2dOscillator()
+    return SweepConfig(
+        'hcp100_sparse', osc, CF.Linear(osc), SC.EulerStep(dt), csr, P,
+        n_steps, params={'c:a': np.tile(G, n_subj)}, group_size=per_subj,
+        seed=6, description='G2DO N=84, %d subjects on one pattern (own '
+        'weights and delays) x %d couplings' % (n_subj, per_subj))
+
+
 def hcp0_kuramoto(path):
     """The reference's hackathon workload (examples/hackathon.py:23-89):
     Kuramoto on data/hcp0.npz with osc.dt = 1.0, omega = 2 pi 10 / 1e3."""
@@ -285,6 +310,7 @@ def hcp0_kuramoto(path):
 
 
 ALL = {'kuramoto76': kuramoto76, 'g2do84': g2do84,
+       'hcp100_sparse': hcp100_sparse,
        'jansenrit998': jansenrit998, 'mpr84': mpr84,
        'jansenrit998_bf16': lambda **kw: jansenrit998_mixed(
            ring_dtype='bf16', **kw),

@@ -20,6 +20,7 @@ raise.
 
 import ctypes
 import os
+import sys
 import threading
 
 import numpy as np
@@ -48,7 +49,8 @@ class SimDesc(ctypes.Structure):
             'scheme', 'tavg_period', 'bold', 'bold_obs', 'fc', 'sample_slots',
             'waves', 'node_block', 'steps_per_launch', 'compat_outputs',
             'order', 'ipl', 'ring_type')] + [('ring_scale', ctypes.c_float),
-                                             ('xcd_map', ctypes.c_int)]
+                                             ('xcd_map', ctypes.c_int),
+                                             ('weight_groups', ctypes.c_int)]
 
 
 class BatchDesc(ctypes.Structure):
@@ -105,10 +107,15 @@ _SIGS = {
     'tvbx_sim_get_samples': [_P, _P, _P, _P],
     'tvbx_sim_fc': [_P, _I, _I, _P, _P],
     'tvbx_sim_fc_reset': [_P],
+    'tvbx_sim_fc_sc_error': [_P, _P, _I, _I, _P, _P],
     'tvbx_sim_get_flags': [_P, _P],
     'tvbx_sim_timing': [_P, _P, _P, _I],
     'tvbx_rng_normals': [_P, ctypes.c_ulonglong, _P, _SZ],
     'tvbx_corr': [_P, _P, _LL, _I, _I, _P],
+    'tvbx_rmap_create': [_P, _P, _I, _I, _P],
+    'tvbx_rmap_destroy': [_P],
+    'tvbx_rmap_to_avg': [_P, _I, _I, _P, _P],
+    'tvbx_rmap_from_avg': [_P, _I, _I, _P, _P],
     'tvbx_batch_create': [_P, _P, ctypes.POINTER(BatchDesc),
                           ctypes.POINTER(SubjectDesc), _P],
     'tvbx_batch_destroy': [_P],
@@ -154,7 +161,7 @@ def lib():
                 fn = getattr(L, name)
                 fn.argtypes = args
                 fn.restype = ctypes.c_int
-            if L.tvbx_abi_version() != 2:
+            if L.tvbx_abi_version() != 3:
                 raise TvbxError('libtvbx ABI version mismatch')
             _lib = L
     return _lib
@@ -256,6 +263,76 @@ class Context:
         out = np.zeros((P, N, N), np.float32)
         check(lib().tvbx_corr(self.h, _ptr(s), n, N, P, _ptr(out)))
         return out
+
+
+class RegionMap:
+    """Surface vertices <-> regions on the device (the reference's RMapToAvg /
+    RMapFromAvg, network.py:174-202).  ``rmap[k]`` is vertex k's region in
+    [0, nroi).  Arrays are (rows, nvar) or (rows, nvar, n_inst): numpy arrays
+    (staged through the device) or torch CUDA float32 tensors (used in
+    place)."""
+
+    def __init__(self, rmap, nroi=None, context=None):
+        rm = np.ascontiguousarray(np.asarray(rmap).reshape(-1), np.int32)
+        if nroi is None:
+            nroi = int(rm.max()) + 1 if rm.size else 0
+        if rm.size and (rm.min() < 0 or rm.max() >= nroi):
+            raise ValueError('rmap entries must lie in [0, %d)' % nroi)
+        self.nnode, self.nroi = int(rm.size), int(nroi)
+        self.ctx = context or Context.default()
+        self.h = ctypes.c_void_p()
+        check(lib().tvbx_rmap_create(self.ctx.h, _ptr(rm), self.nnode,
+                                     self.nroi, ctypes.byref(self.h)))
+
+    def __del__(self):
+        h = getattr(self, 'h', None)
+        if h and _lib is not None:
+            _lib.tvbx_rmap_destroy(h)
+            self.h = None
+
+    def _run(self, fn, rows_in, rows_out, src, dst):
+        torch = sys.modules.get('torch')
+        is_t = torch is not None and isinstance(src, torch.Tensor)
+        shape = tuple(src.shape)
+        if len(shape) not in (2, 3) or shape[0] != rows_in:
+            raise ValueError('expected an array of shape (%d, nvar[, n_inst])'
+                             % rows_in)
+        nvar = shape[1]
+        n_inst = shape[2] if len(shape) == 3 else 1
+        oshape = (rows_out, ) + shape[1:]
+        if is_t:
+            if not src.is_cuda or src.dtype != torch.float32:
+                raise ValueError('torch inputs must be float32 CUDA tensors')
+            src = src.contiguous()
+            if dst is None:
+                dst = torch.empty(oshape, dtype=torch.float32,
+                                  device=src.device)
+            if tuple(dst.shape) != oshape or not dst.is_contiguous():
+                raise ValueError('output must be contiguous, shape %s'
+                                 % (oshape, ))
+            torch.cuda.current_stream(src.device).synchronize()
+            check(fn(self.h, nvar, n_inst, ctypes.c_void_p(src.data_ptr()),
+                     ctypes.c_void_p(dst.data_ptr())))
+            self.ctx.sync()
+            return dst
+        s = np.ascontiguousarray(src, np.float32)
+        out = np.empty(oshape, np.float32) if dst is None else dst
+        if out.shape != oshape or out.dtype != np.float32 or \
+                not out.flags.c_contiguous:
+            raise ValueError('output must be C-contiguous float32, shape %s'
+                             % (oshape, ))
+        check(fn(self.h, nvar, n_inst, _ptr(s), _ptr(out)))
+        return out
+
+    def to_avg(self, node, roi=None):
+        "Region sums roi[r] = sum over rmap[k] == r of node[k]."
+        return self._run(lib().tvbx_rmap_to_avg, self.nnode, self.nroi,
+                         node, roi)
+
+    def from_avg(self, roi, node=None):
+        "node[k] = roi[rmap[k]]."
+        return self._run(lib().tvbx_rmap_from_avg, self.nroi, self.nnode,
+                         roi, node)
 
 
 # ---------------------------------------------------------------------------
@@ -402,6 +479,7 @@ class Simulation:
         d.ring_type = emit.RING_TYPES[ring_dtype]
         d.ring_scale = float(ring_scale)
         d.xcd_map = int(bool(xcd_map))
+        d.weight_groups = int(csr.weight_groups)
         self.desc = d
         self.h = ctypes.c_void_p()
         check(lib().tvbx_sim_create(self.ctx.h, self.module, ctypes.byref(d),
@@ -613,6 +691,21 @@ class Simulation:
 
     def fc_reset(self):
         check(lib().tvbx_sim_fc_reset(self.h))
+
+    def fc_sc_error(self, sc, p0=0, np_=None):
+        """sum over node pairs i < j of (FC_ij - SC_ij)^2 for instances
+        [p0, p0 + np) (FP64, computed on the device from the FC accumulated
+        since the last ``fc_reset``); sc (N, N)"""
+        np_ = self.P - p0 if np_ is None else np_
+        sc = np.ascontiguousarray(sc, np.float32)
+        if sc.shape != (self.N, self.N):
+            raise ValueError('sc must be (%d, %d)' % (self.N, self.N))
+        out = np.zeros(np_, np.float64)
+        ns = ctypes.c_longlong()
+        check(lib().tvbx_sim_fc_sc_error(self.h, _ptr(sc), p0, np_, _ptr(out),
+                                         ctypes.byref(ns)))
+        self.n_fc_samples = ns.value
+        return out
 
     def flags(self):
         f = np.zeros(self.P, np.uint32)

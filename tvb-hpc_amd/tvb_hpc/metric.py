@@ -91,6 +91,25 @@ def windowed_fc(trace, win):
     return out
 
 
+def windowed_fc_sc_error(sim, sc, n_windows, window_steps):
+    """The windowed acceptance statistic of examples/hackathon.py:64-81 on the
+    device: for each of n_windows consecutive windows of window_steps steps,
+    the FC of that window's monitor samples (FP64 co-moments, restarted per
+    window) and sum over i < j of (FC_ij - SC_ij)^2 per instance.  Only the
+    (n_windows, P) errors leave the device.  ``sim`` is a
+    ``hip.Simulation(fc=True)`` whose tavg period divides window_steps."""
+    period = sim.desc.tavg_period
+    if period <= 0 or window_steps % period:
+        raise ValueError('window_steps must be a multiple of the tavg period')
+    sim.fc_reset()               # the first window starts here
+    err = np.zeros((n_windows, sim.P))
+    for w in range(n_windows):
+        sim.run(window_steps)
+        err[w] = sim.fc_sc_error(sc)
+        sim.fc_reset()
+    return err
+
+
 def sweep_trends(err, n_speed, n_coupling):
     """hackathon.py:82-87: the change of the FC-SC error along the speed and
     coupling axes of a (speed outer, coupling inner) sweep grid."""

@@ -173,7 +173,10 @@ def synth_connectome(nnode, density, seed, lengths='uniform', lmin=10.0,
 class CSR:
     """CSR connectome with per-group integer delays, in the layout the device
     kernels consume: ``row`` int32 (N+1), ``col`` int32 (nnz), ``weight``
-    float32 (nnz) shared by all instances, ``delays`` (n_group, nnz) int32."""
+    float32 (nnz) shared by all instances or (n_group, nnz) -- one row per
+    delay group, for subjects that share one sparsity pattern with their own
+    weights and delays (the group-sparse layout of examples/hcp100.py:22-53)
+    -- and ``delays`` (n_group, nnz) int32."""
 
     def __init__(self, nnode, row, col, weight, delays):
         self.nnode = int(nnode)
@@ -195,6 +198,11 @@ class CSR:
         return int(self.delays.shape[0])
 
     @property
+    def weight_groups(self):
+        "True when each delay group has its own weights"
+        return self.weight.ndim == 2
+
+    @property
     def max_delay(self):
         return int(self.delays.max()) if self.delays.size else 0
 
@@ -209,8 +217,9 @@ class CSR:
             raise ValueError('row must be non-decreasing')
         if self.col.size and (self.col.min() < 0 or self.col.max() >= N):
             raise ValueError('col out of range')
-        if self.weight.shape != self.col.shape:
-            raise ValueError('weights must have nnz entries')
+        if self.weight.shape != self.col.shape and \
+                self.weight.shape != (self.delays.shape[0], self.col.size):
+            raise ValueError('weights must be (nnz,) or (n_group, nnz)')
         if self.delays.shape[1] != self.col.size:
             raise ValueError('delays must have nnz entries per group')
         if self.delays.size and self.delays.min() < 0:
@@ -242,12 +251,88 @@ class CSR:
             if b - a > 1:
                 key = np.lexsort((self.col[a:b], self.delays[0, a:b]))
                 perm[a:b] = a + key
-        return CSR(self.nnode, self.row, self.col[perm], self.weight[perm],
-                   self.delays[:, perm])
+        return CSR(self.nnode, self.row, self.col[perm],
+                   self.weight[..., perm], self.delays[:, perm])
 
-    def dense_weights(self):
+    def dense_weights(self, group=0):
+        "dense (N, N) weights (of one group when weights are grouped)"
+        w = self.weight[group] if self.weight_groups else self.weight
         W = np.zeros((self.nnode, self.nnode), np.float64)
         for i in range(self.nnode):
             for e in range(self.row[i], self.row[i + 1]):
-                W[i, self.col[e]] += self.weight[e]
+                W[i, self.col[e]] += w[e]
         return W
+
+    @classmethod
+    def group_sparse(cls, weights, lengths, speeds, dt, policy='trunc'):
+        """Subjects on one sparsity pattern (examples/hcp100.py:22-53):
+        weights, lengths (n_subj, N, N); the pattern is the edges present in
+        every subject; each subject becomes one delay group with its own
+        weights and delays (lengths / speed / dt, one speed per subject or
+        one for all)."""
+        W = np.asarray(weights, np.float64)
+        L = np.asarray(lengths, np.float64)
+        mask = (W != 0).all(axis=0)
+        np.fill_diagonal(mask, False)
+        i, j = np.nonzero(mask)      # row-major: rows ascending, cols within
+        N = W.shape[1]
+        row = np.zeros(N + 1, np.int32)
+        np.add.at(row, i + 1, 1)
+        row = np.cumsum(row).astype(np.int32)
+        sp = np.broadcast_to(np.asarray(speeds, np.float64), (W.shape[0], ))
+        ws = np.stack([w[i, j] for w in W]).astype(np.float32)
+        ds = np.stack([lengths_to_delays(l[i, j], v, dt, policy)
+                       for l, v in zip(L, sp)]).astype(np.int32)
+        return cls(N, row, j.astype(np.int32), ws, ds)
+
+
+class _RMapKernel:
+    """Callable with the reference kernel's keyword arguments (nroi, nvar,
+    nnode, rmap, node, roi); mutates roi (to_avg) or node (from_avg) in
+    place.  A trailing instance axis on node and roi batches instances."""
+
+    def __init__(self, target, to_avg):
+        self.target, self.to_avg = target, to_avg
+        self._maps = {}
+
+    def _map(self, rmap, nroi):
+        from .hip import RegionMap
+        rm = np.ascontiguousarray(np.asarray(rmap).reshape(-1), np.int32)
+        key = (rm.tobytes(), int(nroi))
+        if key not in self._maps:
+            ctx = self.target.context() if hasattr(self.target, 'context') \
+                else None
+            self._maps[key] = RegionMap(rm, nroi, context=ctx)
+        return self._maps[key]
+
+    def __call__(self, nroi, nvar, nnode, rmap, node, roi, **_):
+        rm = np.asarray(rmap).reshape(-1)
+        if rm.size != nnode:
+            raise ValueError('rmap has %d entries for nnode = %d'
+                             % (rm.size, nnode))
+        m = self._map(rm, nroi)
+        if np.shape(node)[:2] != (nnode, nvar) or \
+                np.shape(roi)[:2] != (nroi, nvar):
+            raise ValueError('node must be (nnode, nvar[, n]), roi (nroi, '
+                             'nvar[, n])')
+        if self.to_avg:
+            roi[...] = m.to_avg(np.asarray(node, np.float32))
+        else:
+            node[...] = m.from_avg(np.asarray(roi, np.float32))
+
+
+class RMapToAvg:
+    """Region sums of vertex values: roi[rmap[k], j] += node[k, j] over k
+    ascending, after zeroing roi (network.py:174-190 of the reference; the
+    kernel sums, its name notwithstanding)."""
+
+    def kernel(self, target):
+        return _RMapKernel(target, True)
+
+
+class RMapFromAvg:
+    """Vertex values from region values: node[k, j] = roi[rmap[k], j]
+    (network.py:193-202 of the reference)."""
+
+    def kernel(self, target):
+        return _RMapKernel(target, False)
