@@ -209,6 +209,22 @@ int tvbx_sim_fc_reset(tvbx_sim *sim);
  * the sample series or the matrices leaving the device. */
 int tvbx_sim_fc_sc_error(tvbx_sim *sim, const float *sc, int p0, int np,
                          double *err, long long *n_samples);
+/* Euler-Maruyama log-likelihood of a given trajectory (scheme.py:42-48
+ * euler_maruyama_logp: -(x_n - mu)^2 / (2 sd^2) with mu = x + dt f,
+ * sd = sqrt(dt) g, no normalising constant, as the reference writes it),
+ * summed over state variables, nodes and the n_steps transitions:
+ *   logp[p] = sum_n sum_i sum_s -(x(n+1) - x(n) - dt f(x(n), c(n)))^2
+ *                                 / (2 dt g^2)
+ * traj[n_steps + 1][S][N][P] (host) holds x(step) .. x(step + n_steps);
+ * the state becomes traj[0] and each step takes the next frame (teacher
+ * forcing), so the delayed coupling c(n) is that of the trajectory itself
+ * (its history before traj[0] is the ring as the caller left it, e.g.
+ * tvbx_sim_init_history).  Afterwards the state is traj[n_steps], the step
+ * count has advanced by n_steps and the ring holds the trajectory's
+ * observables.  Frames are uploaded `window` at a time (0: 256).  Needs the
+ * Euler-Maruyama scheme (g > 0). */
+int tvbx_sim_logp(tvbx_sim *sim, const float *traj, long long n_steps,
+                  int window, double *logp);
 /* per-instance flag: 1 if a state variable became non-finite */
 int tvbx_sim_get_flags(tvbx_sim *sim, uint32_t *flags);
 /* device time of the step kernels since the last reset, measured with

@@ -102,6 +102,7 @@ class OracleSpec(ctypes.Structure):
         ('ring_scale', ctypes.c_double),
         ('inst_off', ctypes.c_int),
         ('weight_groups', ctypes.c_int),
+        ('force', ctypes.c_void_p), ('logp', ctypes.c_void_p),
     ]
 
 
@@ -376,8 +377,18 @@ class OracleSim:
 
     # -- stepping ---------------------------------------------------------
 
+    def logp(self, traj):
+        """Euler-Maruyama log density (float64, per instance) of the
+        transitions of traj (n + 1, S, N, P) under this network, with
+        teacher forcing (scheme.py:42-48); the state ends at traj[-1]."""
+        if self.scheme != 1:
+            raise ValueError('the transition density needs scheme em')
+        out = np.zeros(self.P)
+        self.run(np.shape(traj)[0] - 1, force=traj, logp=out)
+        return out
+
     def run(self, n_steps, noise=None, seed=None, traj=False,
-            sample_cap=None):
+            sample_cap=None, force=None, logp=None):
         """Advance n_steps.  ``noise``: injected (n_steps, S, N, P) array,
         or (S, N, P) reused every step; ``seed``: Philox stream.  Returns the
         trajectory (n_steps, S, N, P) if ``traj`` and the monitor samples
@@ -436,6 +447,14 @@ class OracleSim:
         sp.ring_scale = self.ring_scale
         sp.inst_off = self.inst_off
         sp.weight_groups = self.weight_groups
+        if force is not None:
+            # forced steps: frames (n_steps + 1, S, N, P); logp (P,) float64
+            fr = np.ascontiguousarray(force, self.dtype)
+            if fr.shape != (n_steps + 1, self.S, self.N, self.P):
+                raise ValueError('force must be (n_steps + 1, S, N, P)')
+            keep.append(fr)
+            sp.force = _ptr(fr)
+            sp.logp = _ptr(logp)
         f = lib().oracle_run_f64 if self.dtype == np.float64 else \
             lib().oracle_run_f32
         rc = f(ctypes.byref(sp))

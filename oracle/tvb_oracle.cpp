@@ -89,6 +89,10 @@ typedef struct {
   int weight_groups;              // 1: weight is T[n_group][nnz], one row
                                   // per delay group (examples/hcp100.py:
                                   // per-subject sW_data over one pattern)
+  const void *force;              // T[n_steps + 1][S][N][P] or null: forced
+                                  // steps, x(n + 1) taken from frame k + 1
+  double *logp;                   // [P]: += the Euler-Maruyama transition
+                                  // log density of each forced step
 } OracleSpec;
 
 }  // extern "C"
@@ -436,6 +440,12 @@ void run_model(OracleSpec *sp) {
     T *st = sp->order == 1 ? (T *)sp->stale + (size_t)p * 2 * R * N : nullptr;
     T x[8], f[8], g[8], y[8], yp[8], xi[8], c[2];
     long long nsamp = 0;
+    double lp_sum = 0.0;
+    if (sp->force)  // the state starts at frame 0
+      for (int s = 0; s < S; ++s)
+        for (int i = 0; i < N; ++i)
+          xs[(size_t)s * N + i] =
+              ((const T *)sp->force)[((size_t)s * N + i) * P + p];
     for (int k = 0; k < sp->n_steps; ++k) {
       long long n = sp->step0 + k;
       int t = (int)(n % H);
@@ -479,7 +489,22 @@ void run_model(OracleSpec *sp) {
             else
               xi[s] = T(0);
           }
-          for (int s = 0; s < S; ++s) x[s] = x[s] + dt * f[s] + sqrt_dt * xi[s] * g[s];
+          if (sp->force) {
+            // scheme.py:42-48 euler_maruyama_logp: mu = x + dt f,
+            // sd = sqrt(dt) g, log p = -(x_n - mu)^2 / (2 sd^2)
+            const T *fr = (const T *)sp->force +
+                          ((size_t)(k + 1) * S * N + i) * P + p;
+            for (int s = 0; s < S; ++s) {
+              const T mu = x[s] + dt * f[s];
+              const T xs1 = fr[(size_t)s * N * P];
+              const double r = (double)(xs1 - mu);
+              lp_sum -= r * r / (2.0 * (double)dt * (double)g[s] * (double)g[s]);
+              x[s] = xs1;
+            }
+          } else {
+            for (int s = 0; s < S; ++s)
+              x[s] = x[s] + dt * f[s] + sqrt_dt * xi[s] * g[s];
+          }
         } else {
           for (int s = 0; s < S; ++s) x[s] = x[s] + dt * f[s];
         }
@@ -520,6 +545,7 @@ void run_model(OracleSpec *sp) {
       }
     }
     written = std::max(written, nsamp);
+    if (sp->logp) sp->logp[p] += lp_sum;
   }
   sp->samples_written = written;
 }

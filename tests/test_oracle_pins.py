@@ -540,3 +540,44 @@ def test_group_sparse_csr_builder():
         rows = np.repeat(np.arange(N), np.diff(csr.row))
         np.testing.assert_array_equal(
             csr.delays[s], np.floor(L[s][rows, csr.col] / v / 0.1))
+
+
+# ---------------------------------------------- EM transition log density
+
+def test_oracle_logp_of_own_trajectory_is_minus_half_noise_energy():
+    """For a trajectory the float64 Euler-Maruyama recursion generated with
+    injected normals xi, each transition's residual is sqrt(dt) g xi, so
+    the log density of scheme.py:42-48 summed over the run is
+    -sum xi^2 / 2 per instance (up to float64 rounding).  A dropped dt, a
+    wrong sign, or a frame off by one breaks it."""
+    csr = small_csr(N=10, speeds=(2.0, ))
+    P, T, dt, sig = 3, 120, 0.1, 0.02
+    m = MODELS['Generic2dOscillator']()
+    x0 = m.initial_state(10, P, np.random.default_rng(1)).astype(np.float64)
+    xi = np.random.default_rng(2).standard_normal((T, 2, 10, P))
+    kw = dict(scheme='em', cparams={'a': np.full(P, 0.4)}, sigma=sig,
+              dtype=np.float64)
+    gen = O.OracleSim('Generic2dOscillator', 'Sigmoidal', 10, csr.row,
+                      csr.col, csr.weight, csr.delays, P, csr.max_delay + 2,
+                      dt, **kw)
+    gen.set_state(x0)
+    gen.init_history('zeros')
+    tr = gen.run(T, noise=xi, traj=True)['traj']
+    frames = np.concatenate([x0[None], tr])
+    ev = O.OracleSim('Generic2dOscillator', 'Sigmoidal', 10, csr.row,
+                     csr.col, csr.weight, csr.delays, P, csr.max_delay + 2,
+                     dt, **kw)
+    ev.set_state(x0)              # init_history writes h(x0) at slot 0
+    ev.init_history('zeros')
+    lp = ev.logp(frames)
+    np.testing.assert_allclose(lp, -0.5 * (xi ** 2).sum(axis=(0, 1, 2)),
+                               rtol=1e-9)
+    np.testing.assert_array_equal(ev.get_state(), frames[-1])
+    # a frame off by one is far from it
+    lp2 = O.OracleSim('Generic2dOscillator', 'Sigmoidal', 10, csr.row,
+                      csr.col, csr.weight, csr.delays, P, csr.max_delay + 2,
+                      dt, **kw)
+    lp2.set_state(x0)
+    lp2.init_history('zeros')
+    bad = lp2.logp(np.concatenate([frames[:1], frames[:-1]]))
+    assert (bad < 1.5 * lp).all()

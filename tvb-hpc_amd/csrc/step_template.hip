@@ -554,26 +554,45 @@ __device__ __forceinline__ void tvbx_advance(
     tvbx_dfun(x[q], c[q], f, g, lp[q], a, i, p);
 #if TVBX_EM
     float xi[TVBX_S];
-    if (a.noise_mode == 1) {
+    if (a.force != nullptr) {
+      // forced step: the transition log density of the given x(n + 1)
+      // (scheme.py:42-48, mu = x + dt f, sd = sqrt(dt) g), accumulated in
+      // FP64; the state then takes the given value
+      const float *__restrict__ xn =
+          a.force + (size_t)(n + 1 - a.force_step0) * TVBX_S * NP + ip + q;
+      double lp_acc = 0.0;
 #pragma unroll
-      for (int s0 = 0; s0 < TVBX_S; s0 += 4) {
-        float z[4];
-        tvbx_normals4(a, n, i, a.inst_off + p, s0, z);
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (s0 + r < TVBX_S) xi[s0 + r] = z[r];
+      for (int s = 0; s < TVBX_S; ++s) {
+        const float mu = x[q][s] + a.dt * f[s];
+        const float xs1 = xn[(size_t)s * NP];
+        const double r = (double)(xs1 - mu);
+        const double sd2 = (double)a.dt * (double)g[s] * (double)g[s];
+        lp_acc -= r * r / (2.0 * sd2);
+        x[q][s] = xs1;
       }
-    } else if (a.noise_mode == 2) {
-      const float *nz = a.noise + (size_t)k * a.noise_tstride;
-#pragma unroll
-      for (int s = 0; s < TVBX_S; ++s) xi[s] = nz[s * NP + ip + q];
+      a.logp[ip + q] += lp_acc;
     } else {
+      if (a.noise_mode == 1) {
 #pragma unroll
-      for (int s = 0; s < TVBX_S; ++s) xi[s] = 0.0f;
+        for (int s0 = 0; s0 < TVBX_S; s0 += 4) {
+          float z[4];
+          tvbx_normals4(a, n, i, a.inst_off + p, s0, z);
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (s0 + r < TVBX_S) xi[s0 + r] = z[r];
+        }
+      } else if (a.noise_mode == 2) {
+        const float *nz = a.noise + (size_t)k * a.noise_tstride;
+#pragma unroll
+        for (int s = 0; s < TVBX_S; ++s) xi[s] = nz[s * NP + ip + q];
+      } else {
+#pragma unroll
+        for (int s = 0; s < TVBX_S; ++s) xi[s] = 0.0f;
+      }
+#pragma unroll
+      for (int s = 0; s < TVBX_S; ++s)
+        x[q][s] = x[q][s] + a.dt * f[s] + a.sqrt_dt * xi[s] * g[s];
     }
-#pragma unroll
-    for (int s = 0; s < TVBX_S; ++s)
-      x[q][s] = x[q][s] + a.dt * f[s] + a.sqrt_dt * xi[s] * g[s];
 #else
 #pragma unroll
     for (int s = 0; s < TVBX_S; ++s) x[q][s] = x[q][s] + a.dt * f[s];

@@ -105,6 +105,13 @@ typedef struct tvbx_step_args {
   int ring32;
   /* 1: weight holds one row of nnz weights per delay group */
   int weight_groups;
+  /* forced (teacher) steps: when non-null, x(n + 1) is read from
+   * force[(n + 1 - force_step0)][S][N][pitch] instead of being drawn, and
+   * logp[N][pitch] accumulates the Euler-Maruyama transition log density
+   * -(x(n+1) - x(n) - dt f)^2 / (2 dt g^2) of each (node, instance) */
+  const float *force;
+  long long force_step0;
+  double *logp;
 } tvbx_step_args;
 
 /* Argument block of the batched step kernel: per subject (one connectome

@@ -173,6 +173,10 @@ struct tvbx_sim {
   DevBuf out_input, out_drift, out_diffs, tavg, bold_state, samples, bad,
       noise, fc_mean, fc_m2, corr, stale;
   DevBuf fc_tiles, fc_shift, fc_s1, fc_u;  // FC merge work buffers
+  // forced steps (tvbx_sim_logp): trajectory window and log-density sums
+  DevBuf force, logp, logp_sum;
+  long long force_step0 = 0;
+  bool forced_run = false;
   // monitor bookkeeping: sample m is in slot m % sample_slots
   long long samples_made = 0;  // samples written since the monitor started
   long long samples_used = 0;  // samples folded into FC
@@ -729,6 +733,11 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
   }
   a.ring32 = (double)d.horizon * d.n_ring * s->NP * s->esz < 4294967296.0;
   a.weight_groups = d.weight_groups;
+  if (s->forced_run) {
+    a.force = (const float *)s->force.p;
+    a.force_step0 = s->force_step0;
+    a.logp = (double *)s->logp.p;
+  }
   a.ring_unscale = d.ring_type == 0   ? 1.0f
                    : d.ring_type == 3 ? 1.0f / (255.0f * d.ring_scale)
                                       : 1.0f / d.ring_scale;
@@ -869,7 +878,8 @@ extern "C" int tvbx_sim_destroy(tvbx_sim *s) {
                    &s->out_input, &s->out_drift, &s->out_diffs, &s->tavg,
                    &s->bold_state, &s->samples, &s->bad, &s->noise,
                    &s->fc_mean, &s->fc_m2, &s->corr, &s->stale,
-                   &s->fc_tiles, &s->fc_shift, &s->fc_s1, &s->fc_u};
+                   &s->fc_tiles, &s->fc_shift, &s->fc_s1, &s->fc_u,
+                   &s->force, &s->logp, &s->logp_sum};
   for (DevBuf *b : all) dfree(*b);
   dfree(s->clock);
   DevBuf *adj[] = {&s->ccol, &s->crow, &s->cw, &s->lam, &s->mu, &s->rec_x,
@@ -1323,7 +1333,7 @@ extern "C" int tvbx_sim_run(tvbx_sim *s, long long n_steps,
   const unsigned gx = s->pitch / (64 * d.ipl);
   const unsigned gy = (d.n_node + d.node_block - 1) / d.node_block;
   const unsigned bx = 64u * d.waves;
-  const long long W = noise ? 0 : graph_len(s);
+  const long long W = (noise || s->forced_run) ? 0 : graph_len(s);
   long long done = 0;
   while (done < n_steps) {
     if (W > 0 && n_steps - done >= W) {
@@ -1522,6 +1532,56 @@ extern "C" int tvbx_sim_fc_sc_error(tvbx_sim *s, const float *sc, int p0,
   return rc;
 }
 
+// out[p] = sum over nodes i (ascending) of part[i][p].  grid pitch / 64.
+__global__ void k_logp_sum(const double *__restrict__ part, int N, int pitch,
+                           double *__restrict__ out) {
+  const int p = blockIdx.x * 64 + threadIdx.x;
+  double acc = 0.0;
+  for (int i = 0; i < N; ++i) acc += part[(size_t)i * pitch + p];
+  out[p] = acc;
+}
+
+extern "C" int tvbx_sim_logp(tvbx_sim *s, const float *traj,
+                             long long n_steps, int window, double *logp) {
+  CHECK(s && traj && logp, "null argument");
+  CHECK(n_steps >= 1, "need at least one transition");
+  CHECK(s->have_connectome, "set the connectome first");
+  const tvbx_sim_desc &d = s->d;
+  CHECK(d.scheme == 1,
+        "the transition density needs the Euler-Maruyama scheme");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  const long long W = window > 0 ? std::min<long long>(window, n_steps)
+                                 : std::min<long long>(n_steps, 256);
+  const size_t SNP = (size_t)d.n_svar * s->NP;
+  const size_t frame = (size_t)d.n_svar * d.n_node * d.n_inst;
+  int rc = dalloc(s->force, (size_t)(W + 1) * SNP * 4);
+  if (rc == TVBX_OK) rc = dalloc(s->logp, s->NP * 8);
+  if (rc == TVBX_OK) rc = dalloc(s->logp_sum, (size_t)s->pitch * 8);
+  if (rc == TVBX_OK) rc = fill_f32(s, s->force, 0.0f);
+  if (rc == TVBX_OK) rc = fill_f64(s, s->logp, 0.0);
+  if (rc == TVBX_OK) rc = tvbx_sim_set_state(s, traj);
+  for (long long k0 = 0; rc == TVBX_OK && k0 < n_steps; k0 += W) {
+    const long long w = std::min(W, n_steps - k0);
+    // frames x(step) .. x(step + w) of this window
+    rc = up2d(s, s->force.p, traj + (size_t)k0 * frame,
+              (size_t)(w + 1) * d.n_svar * d.n_node);
+    if (rc != TVBX_OK) break;
+    s->force_step0 = s->step;
+    s->forced_run = true;
+    rc = tvbx_sim_run(s, w, nullptr, 0);
+    s->forced_run = false;
+  }
+  if (rc != TVBX_OK) return rc;
+  hipLaunchKernelGGL(k_logp_sum, dim3(s->pitch / 64), dim3(64), 0,
+                     s->ctx->stream, (const double *)s->logp.p, d.n_node,
+                     s->pitch, (double *)s->logp_sum.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(logp, s->logp_sum.p, (size_t)d.n_inst * 8,
+                        hipMemcpyDeviceToHost, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return TVBX_OK;
+}
+
 extern "C" int tvbx_sim_fc_reset(tvbx_sim *s) {
   CHECK(s && s->d.fc, "sim created without fc");
   HIPCHK(hipSetDevice(s->ctx->device));
@@ -1636,7 +1696,7 @@ struct tvbx_rmap {
 // adding into roi[rmap[k]]), so float32 results equal its sequential ones.
 // The loads of RMAP_UNROLL vertices are in flight together and added in
 // order.  grid (nroi * nvar, ceil(n_inst / 64)), block 64.
-#define RMAP_UNROLL 16
+#define RMAP_UNROLL 32
 __global__ void __launch_bounds__(64)
 k_rmap_to_avg(const int *__restrict__ rptr, const int *__restrict__ members,
               int nvar, int n_inst, const float *__restrict__ node,
@@ -1662,22 +1722,22 @@ k_rmap_to_avg(const int *__restrict__ rptr, const int *__restrict__ members,
   if (on) roi[(size_t)rj * n_inst + p] = acc;
 }
 
-// node[k][j][p] = roi[rmap[k]][j][p]: one wave per (k, j) row at a time,
-// rows strided over the grid.  grid (<= 16384, ceil(n_inst / 64)), block
-// 256.
+// node[k][j][p] = roi[rmap[k]][j][p]: one wave per vertex k at a time
+// (all nvar rows), vertices strided over the grid.  grid (<= 16384,
+// ceil(n_inst / 64)), block 256.
 __global__ void __launch_bounds__(256)
 k_rmap_from_avg(const int *__restrict__ rmap, int nvar, int n_inst,
-                long long rows, const float *__restrict__ roi,
+                long long nnode, const float *__restrict__ roi,
                 float *__restrict__ node) {
   const int p = blockIdx.y * 64 + (threadIdx.x & 63);
   const long long w0 = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const long long ws = (long long)gridDim.x * 4;
   if (p >= n_inst) return;
-  for (long long kj = w0; kj < rows; kj += ws) {
-    const long long k = kj / nvar;
-    const int j = (int)(kj - k * nvar);
-    node[(size_t)kj * n_inst + p] =
-        roi[((size_t)rmap[k] * nvar + j) * n_inst + p];
+  const size_t row = (size_t)nvar * n_inst;
+  for (long long k = w0; k < nnode; k += ws) {
+    const float *__restrict__ src = roi + (size_t)rmap[k] * row + p;
+    float *__restrict__ dst = node + (size_t)k * row + p;
+    for (int j = 0; j < nvar; ++j) dst[(size_t)j * n_inst] = src[(size_t)j * n_inst];
   }
 }
 
@@ -1781,10 +1841,10 @@ static int rmap_run(tvbx_rmap *r, bool to_avg, int nvar, int n_inst,
     else
       hipLaunchKernelGGL(k_rmap_from_avg,
                          dim3((unsigned)std::min<long long>(
-                                  (rows + 3) / 4, 16384),
+                                  ((long long)r->nnode + 3) / 4, 16384),
                               tiles),
                          dim3(256), 0, st, (const int *)r->rmap.p, nvar,
-                         n_inst, rows, s, d);
+                         n_inst, (long long)r->nnode, s, d);
     e = hipGetLastError();
   }
   if (rc == TVBX_OK && e == hipSuccess && !ddst)

@@ -108,6 +108,7 @@ _SIGS = {
     'tvbx_sim_fc': [_P, _I, _I, _P, _P],
     'tvbx_sim_fc_reset': [_P],
     'tvbx_sim_fc_sc_error': [_P, _P, _I, _I, _P, _P],
+    'tvbx_sim_logp': [_P, _P, _LL, _I, _P],
     'tvbx_sim_get_flags': [_P, _P],
     'tvbx_sim_timing': [_P, _P, _P, _I],
     'tvbx_rng_normals': [_P, ctypes.c_ulonglong, _P, _SZ],
@@ -691,6 +692,22 @@ class Simulation:
 
     def fc_reset(self):
         check(lib().tvbx_sim_fc_reset(self.h))
+
+    def logp(self, traj, window=0):
+        """Euler-Maruyama log-likelihood per instance (FP64) of a trajectory
+        traj (n + 1, S, N, P): the transitions x(k) -> x(k + 1) with the
+        delayed coupling of the trajectory itself (teacher forcing; the
+        history before traj[0] is the ring as it stands).  Leaves the
+        state at traj[-1] and the step advanced by n (scheme.py:42-48)."""
+        tr = np.ascontiguousarray(traj, np.float32)
+        if tr.ndim != 4 or tr.shape[1:] != (self.spec.S, self.N, self.P) \
+                or tr.shape[0] < 2:
+            raise ValueError('traj must be (n + 1, %d, %d, %d), n >= 1'
+                             % (self.spec.S, self.N, self.P))
+        out = np.zeros(self.P, np.float64)
+        check(lib().tvbx_sim_logp(self.h, _ptr(tr), tr.shape[0] - 1,
+                                  int(window), _ptr(out)))
+        return out
 
     def fc_sc_error(self, sc, p0=0, np_=None):
         """sum over node pairs i < j of (FC_ij - SC_ij)^2 for instances
