@@ -222,9 +222,16 @@ int tvbx_sim_fc_sc_error(tvbx_sim *sim, const float *sc, int p0, int np,
  * tvbx_sim_init_history).  Afterwards the state is traj[n_steps], the step
  * count has advanced by n_steps and the ring holds the trajectory's
  * observables.  Frames are uploaded `window` at a time (0: 256).  Needs the
- * Euler-Maruyama scheme (g > 0). */
+ * Euler-Maruyama scheme (g > 0).
+ * With n_gc / n_gm > 0 (a module emitted with the adjoint) it also returns
+ * the gradients of logp with respect to the parameters the adjoint
+ * differentiates, in the adjoint's layout: g_coupling[n_gc][P] (summed over
+ * nodes) and g_model[n_gm][N][P].  x(n) being given, these need no reverse
+ * recursion: d logp / d f_s = r_s / g_s^2 times the emitted drift
+ * Jacobians (model.py:61-65's symbolic partials), summed in FP64. */
 int tvbx_sim_logp(tvbx_sim *sim, const float *traj, long long n_steps,
-                  int window, double *logp);
+                  int window, double *logp, int n_gc, float *g_coupling,
+                  int n_gm, float *g_model);
 /* per-instance flag: 1 if a state variable became non-finite */
 int tvbx_sim_get_flags(tvbx_sim *sim, uint32_t *flags);
 /* device time of the step kernels since the last reset, measured with

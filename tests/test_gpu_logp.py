@@ -86,3 +86,105 @@ def test_logp_same_across_launch_shapes_and_windows():
     _, b = _device_logp(csr, G, frames, waves=2, node_block=2,
                         steps_per_launch=L)
     np.testing.assert_array_equal(a, b)
+
+
+def test_logp_gradients_match_fp64_finite_differences():
+    """d logp / d G (coupling, per instance) and d logp / d eta (model, per
+    node and instance) of a forced MPR trajectory from the emitted drift
+    Jacobians, against central differences of the float64 oracle's logp
+    (which, x(n) being given, is a smooth function of the parameters)."""
+    from tvb_hpc import coupling, model
+    csr = small_csr(N=12, density=0.5, seed=11, speeds=(1.0, ), dt=DT,
+                    lmax=40.0)
+    n, P2, T2, sig = 12, 64, 150, 0.01
+    m = model.MontbrioPazoRoxin()
+    c = coupling.Linear(m)
+    rng = np.random.default_rng(5)
+    G = rng.uniform(0.2, 0.8, P2)
+    eta = -4.6 + 0.2 * rng.standard_normal((n, P2))
+    x0 = np.stack([rng.uniform(0.1, 1.0, (n, P2)),
+                   rng.uniform(-1.5, -0.5, (n, P2))])
+    xi = rng.standard_normal((T2, 2, n, P2))
+
+    def oracle(Gv, etav, dtype):
+        o = O.OracleSim('MontbrioPazoRoxin', 'Linear', n, csr.row, csr.col,
+                        csr.weight, csr.delays, P2, csr.max_delay + 2, DT,
+                        scheme='em', params={'eta': etav},
+                        cparams={'a': Gv}, sigma=sig, dtype=dtype)
+        o.set_state(x0)
+        o.init_history('zeros')
+        return o
+
+    frames = np.concatenate([x0[None], oracle(G, eta, np.float32).run(
+        T2, noise=xi, traj=True)['traj']]).astype(np.float32)
+    sim = Simulation(m, c, SC.EulerMaryuyamaStep(DT, sigma=sig), csr, P2,
+                     params={'c:a': G.astype(np.float32),
+                             'm:eta': eta.astype(np.float32)},
+                     kinds={'c:a': 'inst', 'm:eta': 'ni'}, adjoint=True,
+                     waves=4, node_block=4)
+    sim.set_state(frames[0])
+    sim.init_history('zeros')
+    lp, g = sim.logp(frames, window=50, grad=True)
+    G32 = G.astype(np.float32).astype(np.float64)
+    eta32 = eta.astype(np.float32).astype(np.float64)
+    f64 = frames.astype(np.float64)
+    ref = oracle(G32, eta32, np.float64).logp(f64)
+    assert np.abs(lp - ref).max() < 1e-3 * np.abs(ref).max()
+    h = 1e-6
+    fd_G = (oracle(G32 + h, eta32, np.float64).logp(f64) -
+            oracle(G32 - h, eta32, np.float64).logp(f64)) / (2 * h)
+    errG = np.abs(g['c:a'] - fd_G).max() / np.abs(fd_G).max()
+    print('d logp / dG vs fp64 central differences %.2e' % errG)
+    assert errG < 2e-3
+    for i in (0, 7):
+        e_p, e_m = eta32.copy(), eta32.copy()
+        e_p[i] += h
+        e_m[i] -= h
+        fd = (oracle(G32, e_p, np.float64).logp(f64) -
+              oracle(G32, e_m, np.float64).logp(f64)) / (2 * h)
+        err = np.abs(g['m:eta'][i] - fd).max() / np.abs(fd).max()
+        print('d logp / d eta[%d] vs fp64 central differences %.2e' % (i, err))
+        assert err < 2e-3
+
+
+def test_logp_autograd_matches_direct_gradients():
+    """autograd.logp: the backward of sum(w * logp) equals w times the
+    direct per-instance gradients."""
+    import torch
+    from tvb_hpc import autograd as AG
+    from tvb_hpc import coupling, model
+    csr = small_csr(N=10, density=0.5, seed=11, speeds=(1.0, ), dt=DT,
+                    lmax=30.0)
+    n, P2, T2, sig = 10, 64, 80, 0.01
+    m = model.MontbrioPazoRoxin()
+    c = coupling.Linear(m)
+    rng = np.random.default_rng(6)
+    G = rng.uniform(0.2, 0.8, P2).astype(np.float32)
+    eta = (-4.6 + 0.2 * rng.standard_normal((n, P2))).astype(np.float32)
+    x0 = np.stack([rng.uniform(0.1, 1.0, (n, P2)),
+                   rng.uniform(-1.5, -0.5, (n, P2))]).astype(np.float32)
+    o = O.OracleSim('MontbrioPazoRoxin', 'Linear', n, csr.row, csr.col,
+                    csr.weight, csr.delays, P2, csr.max_delay + 2, DT,
+                    scheme='em', params={'eta': eta}, cparams={'a': G},
+                    sigma=sig, dtype=np.float32)
+    o.set_state(x0)
+    o.init_history('zeros')
+    frames = np.concatenate([x0[None], o.run(T2, seed=3, traj=True)['traj']])
+    sim = Simulation(m, c, SC.EulerMaryuyamaStep(DT, sigma=sig), csr, P2,
+                     params={'c:a': G, 'm:eta': eta},
+                     kinds={'c:a': 'inst', 'm:eta': 'ni'}, adjoint=True,
+                     waves=4, node_block=4)
+    Gt = torch.tensor(G, dtype=torch.float64, requires_grad=True)
+    et = torch.tensor(eta, dtype=torch.float64, requires_grad=True)
+    w = torch.linspace(0.5, 1.5, P2, dtype=torch.float64)
+    lp = AG.logp(sim, {'c:a': Gt, 'm:eta': et}, frames)
+    (w * lp).sum().backward()
+    sim.step = 0
+    sim.set_state(frames[0])
+    sim.init_history('zeros')
+    lp2, g = sim.logp(frames, grad=True)
+    np.testing.assert_array_equal(lp.detach().numpy(), lp2)
+    np.testing.assert_allclose(Gt.grad.numpy(), w.numpy() * g['c:a'],
+                               rtol=1e-12)
+    np.testing.assert_allclose(et.grad.numpy(), w.numpy() * g['m:eta'],
+                               rtol=1e-12)

@@ -175,3 +175,73 @@ tvbx_adjoint(const tvbx_adj_args A) {
     if (k + 1 < a.n_steps) __syncthreads();
   }
 }
+
+// Parameter gradients of the Euler-Maruyama transition log density of
+// forced steps (scheme.py:42-48; no recursion: x(n) is given).  With
+// r = x(n+1) - x(n) - dt f and log p = -r^2 / (2 dt g^2) per state
+// variable, d log p / d f_s = r_s / g_s^2 =: lam_s, so
+//   d log p / d theta_m = sum_s lam_s d f_s / d theta_m        (model)
+//   d log p / d theta_c = [sum_s lam_s d f_s / d c] d post / d theta_c
+// with the emitted vector-Jacobian products; c(n) = post(sum(n)) from the
+// sums the forced steps recorded.  FP64 accumulation over the window.
+// grid (pitch / (64 ipl), N), block 64.
+extern "C" __global__ void __launch_bounds__(64)
+tvbx_logp_grad(const tvbx_logp_args L) {
+  const tvbx_step_args &a = L.f;
+  const int pb = (blockIdx.x * 64 + (int)threadIdx.x) * TVBX_IPL;
+  const int i = blockIdx.y;
+  const size_t NP = (size_t)a.n_node * a.pitch;
+  const size_t ip = (size_t)i * a.pitch + pb;
+  tvbx_lane_params lp[TVBX_IPL];
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q) tvbx_load_lane(lp[q], a, pb + q);
+  double gm_acc[TVBX_IPL][TVBX_NGM], gc_acc[TVBX_IPL][TVBX_NGC];
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q) {
+#pragma unroll
+    for (int g = 0; g < TVBX_NGM; ++g) gm_acc[q][g] = 0.0;
+#pragma unroll
+    for (int g = 0; g < TVBX_NGC; ++g) gc_acc[q][g] = 0.0;
+  }
+  for (int k = 0; k < a.n_steps; ++k) {
+    const float *__restrict__ x0 = a.force + (size_t)k * TVBX_S * NP + ip;
+    const float *__restrict__ x1 = x0 + (size_t)TVBX_S * NP;
+    const float *__restrict__ sm = L.rec_sum + (size_t)k * TVBX_C * NP + ip;
+#pragma unroll
+    for (int q = 0; q < TVBX_IPL; ++q) {
+      const int p = pb + q;
+      float x[TVBX_S], f[TVBX_S], g[TVBX_S], lam[TVBX_S], jt[TVBX_S],
+          c[TVBX_C], muc[TVBX_C], gm[TVBX_NGM], gc[TVBX_NGC];
+#pragma unroll
+      for (int s = 0; s < TVBX_S; ++s) x[s] = x0[(size_t)s * NP + q];
+      const float sum = sm[q];
+      c[0] = tvbx_post(0, sum, lp[q], a);
+      tvbx_dfun(x, c, f, g, lp[q], a, i, p);
+#pragma unroll
+      for (int s = 0; s < TVBX_S; ++s) {
+        const float mu = x[s] + a.dt * f[s];
+        lam[s] = (x1[(size_t)s * NP + q] - mu) / (g[s] * g[s]);
+      }
+      tvbx_dfun_vjp(x, c, lam, jt, muc, gm, lp[q], a, i, p);
+      tvbx_dpost_dparam(sum, gc, lp[q], a);
+#pragma unroll
+      for (int r = 0; r < TVBX_NGM; ++r) gm_acc[q][r] += (double)gm[r];
+#pragma unroll
+      for (int r = 0; r < TVBX_NGC; ++r)
+        gc_acc[q][r] += (double)muc[0] * (double)gc[r];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q) {
+#if TVBX_HAS_NGM
+#pragma unroll
+    for (int r = 0; r < TVBX_NGM; ++r)
+      L.g_m[(size_t)r * NP + ip + q] += gm_acc[q][r];
+#endif
+#if TVBX_HAS_NGC
+#pragma unroll
+    for (int r = 0; r < TVBX_NGC; ++r)
+      L.g_c[(size_t)r * NP + ip + q] += gc_acc[q][r];
+#endif
+  }
+}

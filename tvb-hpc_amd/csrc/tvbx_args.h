@@ -147,6 +147,16 @@ typedef struct tvbx_adj_args {
   float *g_m;             /* [n_gm][N][pitch] model-parameter partials */
 } tvbx_adj_args;
 
+/* Parameter gradients of the Euler-Maruyama log density of forced steps:
+ * frames f.force[k][S][N][pitch] for k = 0 .. f.n_steps and the coupling
+ * sums those steps recorded */
+typedef struct tvbx_logp_args {
+  tvbx_step_args f;
+  const float *rec_sum;   /* [n_steps][C][N][pitch] */
+  double *g_c;            /* [n_gc][N][pitch] += d logp / d coupling param */
+  double *g_m;            /* [n_gm][N][pitch] += d logp / d model param */
+} tvbx_logp_args;
+
 #ifdef __cplusplus
 }
 #endif

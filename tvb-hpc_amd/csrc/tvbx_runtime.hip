@@ -149,6 +149,7 @@ struct tvbx_module {
   hipFunction_t adj = nullptr;  // present when emitted with the adjoint
   hipFunction_t blocked = nullptr;  // absent in the reference ordering
   hipFunction_t batch = nullptr;    // absent in the reference ordering
+  hipFunction_t logp_grad = nullptr;  // present when emitted with the adjoint
   std::vector<char> image;
 };
 
@@ -174,9 +175,10 @@ struct tvbx_sim {
       noise, fc_mean, fc_m2, corr, stale;
   DevBuf fc_tiles, fc_shift, fc_s1, fc_u;  // FC merge work buffers
   // forced steps (tvbx_sim_logp): trajectory window and log-density sums
-  DevBuf force, logp, logp_sum;
+  DevBuf force, logp, logp_sum, lg_c, lg_m;
   long long force_step0 = 0;
   bool forced_run = false;
+  bool forced_rec = false;  // forced steps record their coupling sums
   // monitor bookkeeping: sample m is in slot m % sample_slots
   long long samples_made = 0;  // samples written since the monitor started
   long long samples_used = 0;  // samples folded into FC
@@ -308,6 +310,9 @@ static int module_from_image(tvbx_ctx *c, const void *image, size_t size,
   }
   if (hipModuleGetFunction(&m->adj, m->mod, "tvbx_adjoint") != hipSuccess)
     m->adj = nullptr;
+  if (hipModuleGetFunction(&m->logp_grad, m->mod, "tvbx_logp_grad") !=
+      hipSuccess)
+    m->logp_grad = nullptr;
   if (hipModuleGetFunction(&m->blocked, m->mod, "tvbx_step_blocked") !=
       hipSuccess)
     m->blocked = nullptr;
@@ -737,6 +742,10 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
     a.force = (const float *)s->force.p;
     a.force_step0 = s->force_step0;
     a.logp = (double *)s->logp.p;
+    if (s->forced_rec) {
+      a.rec_sum = (float *)s->rec_sum.p;
+      a.rec_step0 = s->force_step0;
+    }
   }
   a.ring_unscale = d.ring_type == 0   ? 1.0f
                    : d.ring_type == 3 ? 1.0f / (255.0f * d.ring_scale)
@@ -879,7 +888,7 @@ extern "C" int tvbx_sim_destroy(tvbx_sim *s) {
                    &s->bold_state, &s->samples, &s->bad, &s->noise,
                    &s->fc_mean, &s->fc_m2, &s->corr, &s->stale,
                    &s->fc_tiles, &s->fc_shift, &s->fc_s1, &s->fc_u,
-                   &s->force, &s->logp, &s->logp_sum};
+                   &s->force, &s->logp, &s->logp_sum, &s->lg_c, &s->lg_m};
   for (DevBuf *b : all) dfree(*b);
   dfree(s->clock);
   DevBuf *adj[] = {&s->ccol, &s->crow, &s->cw, &s->lam, &s->mu, &s->rec_x,
@@ -1542,23 +1551,41 @@ __global__ void k_logp_sum(const double *__restrict__ part, int N, int pitch,
 }
 
 extern "C" int tvbx_sim_logp(tvbx_sim *s, const float *traj,
-                             long long n_steps, int window, double *logp) {
+                             long long n_steps, int window, double *logp,
+                             int n_gc, float *g_coupling, int n_gm,
+                             float *g_model) {
   CHECK(s && traj && logp, "null argument");
   CHECK(n_steps >= 1, "need at least one transition");
   CHECK(s->have_connectome, "set the connectome first");
   const tvbx_sim_desc &d = s->d;
   CHECK(d.scheme == 1,
         "the transition density needs the Euler-Maruyama scheme");
+  const bool grads = n_gc > 0 || n_gm > 0;
+  CHECK(n_gc >= 0 && n_gc <= d.n_pinst, "n_gc exceeds the instance parameters");
+  CHECK(n_gm >= 0 && n_gm <= d.n_pinst + d.n_pni,
+        "n_gm exceeds the instance / node-instance parameters");
+  CHECK(!grads || s->mod->logp_grad,
+        "gradients need a module emitted with the adjoint");
+  CHECK(n_gc == 0 || g_coupling, "null g_coupling");
+  CHECK(n_gm == 0 || g_model, "null g_model");
   HIPCHK(hipSetDevice(s->ctx->device));
   const long long W = window > 0 ? std::min<long long>(window, n_steps)
                                  : std::min<long long>(n_steps, 256);
   const size_t SNP = (size_t)d.n_svar * s->NP;
   const size_t frame = (size_t)d.n_svar * d.n_node * d.n_inst;
+  const int ngc = std::max(n_gc, 1), ngm = std::max(n_gm, 1);
   int rc = dalloc(s->force, (size_t)(W + 1) * SNP * 4);
   if (rc == TVBX_OK) rc = dalloc(s->logp, s->NP * 8);
   if (rc == TVBX_OK) rc = dalloc(s->logp_sum, (size_t)s->pitch * 8);
   if (rc == TVBX_OK) rc = fill_f32(s, s->force, 0.0f);
   if (rc == TVBX_OK) rc = fill_f64(s, s->logp, 0.0);
+  if (grads) {
+    if (rc == TVBX_OK) rc = dalloc(s->rec_sum, (size_t)W * d.n_coupled * s->NP * 4);
+    if (rc == TVBX_OK) rc = dalloc(s->lg_c, (size_t)ngc * s->NP * 8);
+    if (rc == TVBX_OK) rc = dalloc(s->lg_m, (size_t)ngm * s->NP * 8);
+    if (rc == TVBX_OK) rc = fill_f64(s, s->lg_c, 0.0);
+    if (rc == TVBX_OK) rc = fill_f64(s, s->lg_m, 0.0);
+  }
   if (rc == TVBX_OK) rc = tvbx_sim_set_state(s, traj);
   for (long long k0 = 0; rc == TVBX_OK && k0 < n_steps; k0 += W) {
     const long long w = std::min(W, n_steps - k0);
@@ -1568,8 +1595,25 @@ extern "C" int tvbx_sim_logp(tvbx_sim *s, const float *traj,
     if (rc != TVBX_OK) break;
     s->force_step0 = s->step;
     s->forced_run = true;
+    s->forced_rec = grads;
     rc = tvbx_sim_run(s, w, nullptr, 0);
     s->forced_run = false;
+    s->forced_rec = false;
+    if (rc != TVBX_OK || !grads) continue;
+    tvbx_logp_args L;
+    memset(&L, 0, sizeof L);
+    fill_args(s, L.f);
+    L.f.force = (const float *)s->force.p;
+    L.f.n_steps = (int)w;
+    L.rec_sum = (const float *)s->rec_sum.p;
+    L.g_c = (double *)s->lg_c.p;
+    L.g_m = (double *)s->lg_m.p;
+    void *params[] = {&L};
+    const hipError_t e = hipModuleLaunchKernel(
+        s->mod->logp_grad, s->pitch / (64 * d.ipl), d.n_node, 1, 64, 1, 1, 0,
+        s->ctx->stream, params, nullptr);
+    if (e != hipSuccess)
+      rc = fail(TVBX_E_HIP, "tvbx_logp_grad: %s", hipGetErrorString(e));
   }
   if (rc != TVBX_OK) return rc;
   hipLaunchKernelGGL(k_logp_sum, dim3(s->pitch / 64), dim3(64), 0,
@@ -1578,7 +1622,30 @@ extern "C" int tvbx_sim_logp(tvbx_sim *s, const float *traj,
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(logp, s->logp_sum.p, (size_t)d.n_inst * 8,
                         hipMemcpyDeviceToHost, s->ctx->stream));
+  std::vector<double> hc, hm;
+  if (grads) {
+    hc.resize((size_t)ngc * s->NP);
+    hm.resize((size_t)ngm * s->NP);
+    HIPCHK(hipMemcpyAsync(hc.data(), s->lg_c.p, hc.size() * 8,
+                          hipMemcpyDeviceToHost, s->ctx->stream));
+    HIPCHK(hipMemcpyAsync(hm.data(), s->lg_m.p, hm.size() * 8,
+                          hipMemcpyDeviceToHost, s->ctx->stream));
+  }
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  const int N = d.n_node, P = d.n_inst;
+  // coupling parameters (kind 'inst'): summed over nodes in ascending order
+  for (int g = 0; g < n_gc; ++g)
+    for (int p = 0; p < P; ++p) {
+      double acc = 0.0;
+      for (int i = 0; i < N; ++i)
+        acc += hc[((size_t)g * N + i) * s->pitch + p];
+      g_coupling[(size_t)g * P + p] = (float)acc;
+    }
+  for (int g = 0; g < n_gm; ++g)
+    for (int i = 0; i < N; ++i)
+      for (int p = 0; p < P; ++p)
+        g_model[((size_t)g * N + i) * P + p] =
+            (float)hm[((size_t)g * N + i) * s->pitch + p];
   return TVBX_OK;
 }
 

@@ -10,6 +10,8 @@ recursion (csrc/adjoint_template.hip), returning dL/dparam for the tensors in
 (N, P)).  Any differentiable torch loss of the samples can sit on top, e.g.
 ``fc_loss`` below (1 - Pearson correlation of simulated and target FC, the
 Bayesian-inversion use the reference names in README.md:3-4).
+``logp(sim, params, traj)`` is the Euler-Maruyama log-likelihood of a given
+trajectory (scheme.py:42-48) with its parameter gradients.
 
 PyTorch is used only for the autograd plumbing and the loss; the forward and
 reverse sweeps are the HIP kernels behind libtvbx.
@@ -70,6 +72,46 @@ def integrate(sim, params, x0, n_steps, window, seed=None):
     keys = list(params)
     return _Integrate.apply(sim, keys, x0, int(n_steps), int(window), seed,
                             *[params[k] for k in keys])
+
+
+class _Logp(torch.autograd.Function):
+
+    @staticmethod
+    def forward(ctx, sim, keys, traj, window, *values):
+        params = {k: v.detach().double().cpu().numpy()
+                  for k, v in zip(keys, values)}
+        sim.set_params(params)
+        sim.step = 0
+        sim.set_state(traj[0])
+        sim.init_history('zeros')
+        lp, g = sim.logp(traj, window=window, grad=True)
+        ctx.grads = g
+        ctx.keys = keys
+        ctx.device = values[0].device
+        ctx.dtypes = [v.dtype for v in values]
+        return torch.from_numpy(lp).to(ctx.device)
+
+    @staticmethod
+    def backward(ctx, d_logp):
+        # logp[p] depends on instance p's parameters only
+        go = d_logp.detach().double().cpu().numpy()
+        grads = []
+        for k, dt in zip(ctx.keys, ctx.dtypes):
+            g = np.asarray(ctx.grads[k], np.float64) * go
+            grads.append(torch.from_numpy(g).to(ctx.device, dt))
+        return (None, None, None, None) + tuple(grads)
+
+
+def logp(sim, params, traj, window=0):
+    """Euler-Maruyama log-likelihood (P,) of the trajectory traj
+    (n + 1, S, N, P) under ``sim`` (scheme.py:42-48, teacher forcing, history
+    zeros before traj[0]), differentiable w.r.t. the tensors in ``params``
+    (keys and shapes as in ``integrate``; ``sim`` created with
+    adjoint=True)."""
+    keys = list(params)
+    tr = np.ascontiguousarray(traj, np.float32)
+    return _Logp.apply(sim, keys, tr, int(window),
+                       *[params[k] for k in keys])
 
 
 def fc(samples):

@@ -108,7 +108,7 @@ _SIGS = {
     'tvbx_sim_fc': [_P, _I, _I, _P, _P],
     'tvbx_sim_fc_reset': [_P],
     'tvbx_sim_fc_sc_error': [_P, _P, _I, _I, _P, _P],
-    'tvbx_sim_logp': [_P, _P, _LL, _I, _P],
+    'tvbx_sim_logp': [_P, _P, _LL, _I, _P, _I, _P, _I, _P],
     'tvbx_sim_get_flags': [_P, _P],
     'tvbx_sim_timing': [_P, _P, _P, _I],
     'tvbx_rng_normals': [_P, ctypes.c_ulonglong, _P, _SZ],
@@ -693,21 +693,40 @@ class Simulation:
     def fc_reset(self):
         check(lib().tvbx_sim_fc_reset(self.h))
 
-    def logp(self, traj, window=0):
+    def logp(self, traj, window=0, grad=False):
         """Euler-Maruyama log-likelihood per instance (FP64) of a trajectory
         traj (n + 1, S, N, P): the transitions x(k) -> x(k + 1) with the
         delayed coupling of the trajectory itself (teacher forcing; the
         history before traj[0] is the ring as it stands).  Leaves the
-        state at traj[-1] and the step advanced by n (scheme.py:42-48)."""
+        state at traj[-1] and the step advanced by n (scheme.py:42-48).
+        With ``grad`` (a simulation created with adjoint=True) returns
+        (logp, {param key: d logp / d param}) for the parameters the
+        adjoint differentiates: coupling (P,), model 'ni' (N, P) and
+        'inst' (P,)."""
         tr = np.ascontiguousarray(traj, np.float32)
         if tr.ndim != 4 or tr.shape[1:] != (self.spec.S, self.N, self.P) \
                 or tr.shape[0] < 2:
             raise ValueError('traj must be (n + 1, %d, %d, %d), n >= 1'
                              % (self.spec.S, self.N, self.P))
         out = np.zeros(self.P, np.float64)
+        gc, gm = self.spec.grad_params() if grad else ([], [])
+        if grad and not (gc or gm):
+            raise ValueError('no parameter of kind inst / ni to '
+                             'differentiate')
+        g_c = np.zeros((max(len(gc), 1), self.P), np.float32)
+        g_m = np.zeros((max(len(gm), 1), self.N, self.P), np.float32)
         check(lib().tvbx_sim_logp(self.h, _ptr(tr), tr.shape[0] - 1,
-                                  int(window), _ptr(out)))
-        return out
+                                  int(window), _ptr(out), len(gc), _ptr(g_c),
+                                  len(gm), _ptr(g_m)))
+        if not grad:
+            return out
+        g = {}
+        for k, key in enumerate(gc):
+            g[key] = g_c[k]
+        for k, key in enumerate(gm):
+            g[key] = g_m[k] if self.spec.kinds[key] == 'ni' else \
+                g_m[k].sum(axis=0)
+        return out, g
 
     def fc_sc_error(self, sc, p0=0, np_=None):
         """sum over node pairs i < j of (FC_ij - SC_ij)^2 for instances
