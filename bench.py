@@ -317,6 +317,94 @@ def bench_rmap(args):
     print(json.dumps(out), flush=True)
 
 
+LOGP_WORKLOADS = {'logp_mpr84': 'mpr84'}
+
+
+def bench_logp(args):
+    """Euler-Maruyama log-likelihood of a device-resident MPR trajectory
+    (N=84, P=4096, teacher forcing) and its parameter gradients."""
+    import numpy as np
+    import torch
+    from tvb_hpc import autotune, configs
+    cfg = configs.ALL[LOGP_WORKLOADS[args.workload]]()
+    T = args.steps or 1000
+    N, P = cfg.csr.nnode, cfg.n_inst
+    la = dict(autotune.best_launch(cfg))
+    sim = cfg.simulation(adjoint=True, **la)
+    S = sim.spec.S
+    x0 = cfg.initial_state()
+    frames = torch.empty((T + 1, S, N, P), dtype=torch.float32, device='cuda')
+    sim.set_state(x0)
+    sim.init_history('zeros')
+    sim.set_noise(seed=cfg.seed + 1)
+    sim.state_into(frames[0].data_ptr())
+    for k in range(T):
+        sim.run(1)
+        sim.state_into(frames[k + 1].data_ptr())
+    sim.ctx.sync()
+    res = {}
+    for mode in ('logp', 'grad'):
+        for rep in range(1 + max(1, args.warmup // 100)):
+            sim.step = 0
+            sim.set_state(x0)
+            sim.init_history('zeros')
+            t0 = time.perf_counter()
+            out = sim.logp(frames, grad=(mode == 'grad'))
+            el = time.perf_counter() - t0
+        res[mode] = el
+    lp = out[0]
+    nnz = cfg.csr.nnz
+    bytes_step = 4 * nnz * P + 4 * S * N * P + 4 * N * P + 16 * N * P
+    ach = bytes_step * T / res['logp'] / 1e9
+    out = {
+        'metric': 'EM log-likelihood transition*instances/s',
+        'value': P * T / res['logp'], 'unit': 'transition*instance/s',
+        'n_gpus': 1, 'steps': T, 'warmup': args.warmup,
+        'ms_per_step': res['logp'] / T * 1e3, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32 state, f64 sums', 'data': 'synthetic',
+        'config': {'workload': '%s: EM log-likelihood of a %d-transition '
+                               'trajectory of %s' % (args.workload, T,
+                                                     cfg.description),
+                   'launch': la},
+        'roofline': {'bound': 'hbm', 'achieved': ach, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': ach / HBM_PEAK_GBS,
+                     'traffic': None, 'kernel': 'tvbx_step (forced)',
+                     'bytes_per_step': bytes_step,
+                     'note': 'wall time of the whole call'},
+        'with_gradients': {'value': P * T / res['grad'],
+                           'ms_per_step': res['grad'] / T * 1e3},
+        'logp_mean': float(np.mean(lp)),
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        import oracle as O
+        n_c = min(args.cpu_inst, P)
+        idx = np.arange(n_c)
+        fr = frames[:, :, :, :n_c].cpu().numpy()
+        threads = min(16, os.cpu_count() or 1)
+        csr = cfg.csr
+        o = O.OracleSim('MontbrioPazoRoxin', 'Linear', N, csr.row, csr.col,
+                        csr.weight, csr.delays, n_c, csr.max_delay + 2,
+                        cfg.scheme.dt, scheme='em',
+                        cparams={'a': np.asarray(cfg.params['c:a'])[idx]},
+                        sigma=cfg.scheme.sigma, dtype=np.float32,
+                        n_threads=threads)
+        o.set_state(fr[0])
+        o.init_history('zeros')
+        t0 = time.perf_counter()
+        ref = o.logp(fr)
+        el = time.perf_counter() - t0
+        out['cpu_baseline'] = {
+            'value': n_c * T / el, 'unit': 'transition*instance/s',
+            'cores': threads, 'kind': 'port',
+            'sample': '%d instances x %d transitions (float32 oracle)'
+                      % (n_c, T)}
+        out['max_rel_err_vs_oracle'] = float(
+            np.abs(lp[:n_c] - ref).max() / np.abs(ref).max())
+    print(json.dumps(out), flush=True)
+
+
 def bench_batch(args):
     """The uneven-size sweep: subjects with different connectomes in one
     launch per block of steps."""
@@ -473,6 +561,8 @@ def main():
         return bench_batch(args)
     if args.workload in RMAP_WORKLOADS:
         return bench_rmap(args)
+    if args.workload in LOGP_WORKLOADS:
+        return bench_logp(args)
     gw = 0 if args.no_graph else args.graph_window
     cfg = configs.ALL[args.workload]()
     K = args.steps or cfg.n_steps

@@ -153,7 +153,8 @@ int tvbx_sim_set_connectome(tvbx_sim *sim, const int32_t *row,
  * tmpl8_CUDA.py:58 params(i) per work item). */
 int tvbx_sim_set_params(tvbx_sim *sim, int kind, const float *values,
                         size_t count);
-/* state x[S][N][P] (model.py:75-80; cuda_run.py:79) */
+/* state x[S][N][P] in host or device memory (model.py:75-80;
+ * cuda_run.py:79) */
 int tvbx_sim_set_state(tvbx_sim *sim, const float *x);
 int tvbx_sim_get_state(tvbx_sim *sim, float *x);
 /* observable ring ring[H][R][N][P] (the reference's obsrv[ntime][nnode][nobs]
@@ -215,7 +216,9 @@ int tvbx_sim_fc_sc_error(tvbx_sim *sim, const float *sc, int p0, int np,
  * summed over state variables, nodes and the n_steps transitions:
  *   logp[p] = sum_n sum_i sum_s -(x(n+1) - x(n) - dt f(x(n), c(n)))^2
  *                                 / (2 dt g^2)
- * traj[n_steps + 1][S][N][P] (host) holds x(step) .. x(step + n_steps);
+ * traj[n_steps + 1][S][N][P] (host or device memory; device frames with
+ * P a multiple of 64 * ipl are read in place) holds x(step) ..
+ * x(step + n_steps);
  * the state becomes traj[0] and each step takes the next frame (teacher
  * forcing), so the delayed coupling c(n) is that of the trajectory itself
  * (its history before traj[0] is the ring as the caller left it, e.g.

@@ -188,3 +188,27 @@ def test_logp_autograd_matches_direct_gradients():
                                rtol=1e-12)
     np.testing.assert_allclose(et.grad.numpy(), w.numpy() * g['m:eta'],
                                rtol=1e-12)
+
+
+def test_logp_device_frames_in_place_match_host_frames():
+    """frames in device memory (a torch CUDA tensor, P a multiple of 64) are
+    read in place; the result equals the host-frame path's"""
+    import torch
+    csr = small_csr(N=N, dt=DT, speeds=(2.0, ))
+    rng = np.random.default_rng(12)
+    G = rng.uniform(0.0, 0.8, P).astype(np.float32)
+    x0 = MODELS['Generic2dOscillator']().initial_state(
+        N, P, np.random.default_rng(13)).astype(np.float32)
+    xi = rng.standard_normal((T, 2, N, P)).astype(np.float32)
+    frames = _frames(csr, G, x0, xi)
+    _, a = _device_logp(csr, G, frames, waves=4, node_block=4)
+    sim, _ = _device_logp(csr, G, frames[:2], waves=4, node_block=4)
+    sim.step = 0
+    sim.set_state(frames[0])
+    sim.init_history('zeros')
+    b = sim.logp(torch.from_numpy(frames).cuda())
+    np.testing.assert_array_equal(a, b)
+    # the state can be copied into device memory as well
+    out = torch.zeros((2, N, P), device='cuda')
+    sim.state_into(out.data_ptr())
+    np.testing.assert_array_equal(out.cpu().numpy(), frames[-1])

@@ -179,6 +179,7 @@ struct tvbx_sim {
   long long force_step0 = 0;
   bool forced_run = false;
   bool forced_rec = false;  // forced steps record their coupling sums
+  const float *force_ext = nullptr;  // frames read in place (device memory)
   // monitor bookkeeping: sample m is in slot m % sample_slots
   long long samples_made = 0;  // samples written since the monitor started
   long long samples_used = 0;  // samples folded into FC
@@ -670,10 +671,22 @@ static int fill_f64(tvbx_sim *s, DevBuf &b, double v) {
 }
 
 // host [rows][P] <-> device [rows][pitch]
+// true when p is device memory that kernels can read and write directly
+static bool on_device(const void *p) {
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice;
+}
+
+// [rows][P] float32 arrays in host or device memory (the copy kind is
+// inferred from the addresses) <-> pitched device rows
 static int up2d(tvbx_sim *s, void *dst, const float *src, size_t rows) {
   const int P = s->d.n_inst;
   HIPCHK(hipMemcpy2DAsync(dst, (size_t)s->pitch * 4, src, (size_t)P * 4,
-                          (size_t)P * 4, rows, hipMemcpyHostToDevice,
+                          (size_t)P * 4, rows, hipMemcpyDefault,
                           s->ctx->stream));
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
   return TVBX_OK;
@@ -682,7 +695,7 @@ static int up2d(tvbx_sim *s, void *dst, const float *src, size_t rows) {
 static int down2d(tvbx_sim *s, float *dst, const void *src, size_t rows) {
   const int P = s->d.n_inst;
   HIPCHK(hipMemcpy2DAsync(dst, (size_t)P * 4, src, (size_t)s->pitch * 4,
-                          (size_t)P * 4, rows, hipMemcpyDeviceToHost,
+                          (size_t)P * 4, rows, hipMemcpyDefault,
                           s->ctx->stream));
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
   return TVBX_OK;
@@ -739,7 +752,7 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
   a.ring32 = (double)d.horizon * d.n_ring * s->NP * s->esz < 4294967296.0;
   a.weight_groups = d.weight_groups;
   if (s->forced_run) {
-    a.force = (const float *)s->force.p;
+    a.force = s->force_ext ? s->force_ext : (const float *)s->force.p;
     a.force_step0 = s->force_step0;
     a.logp = (double *)s->logp.p;
     if (s->forced_rec) {
@@ -1574,7 +1587,9 @@ extern "C" int tvbx_sim_logp(tvbx_sim *s, const float *traj,
   const size_t SNP = (size_t)d.n_svar * s->NP;
   const size_t frame = (size_t)d.n_svar * d.n_node * d.n_inst;
   const int ngc = std::max(n_gc, 1), ngm = std::max(n_gm, 1);
-  int rc = dalloc(s->force, (size_t)(W + 1) * SNP * 4);
+  int rc = dalloc(s->force, d.n_inst == s->pitch && on_device(traj)
+                                ? 4
+                                : (size_t)(W + 1) * SNP * 4);
   if (rc == TVBX_OK) rc = dalloc(s->logp, s->NP * 8);
   if (rc == TVBX_OK) rc = dalloc(s->logp_sum, (size_t)s->pitch * 8);
   if (rc == TVBX_OK) rc = fill_f32(s, s->force, 0.0f);
@@ -1587,23 +1602,31 @@ extern "C" int tvbx_sim_logp(tvbx_sim *s, const float *traj,
     if (rc == TVBX_OK) rc = fill_f64(s, s->lg_m, 0.0);
   }
   if (rc == TVBX_OK) rc = tvbx_sim_set_state(s, traj);
+  // frames already in device memory with the pitched layout are read in
+  // place; others are copied in windows
+  const bool in_place = d.n_inst == s->pitch && on_device(traj);
   for (long long k0 = 0; rc == TVBX_OK && k0 < n_steps; k0 += W) {
     const long long w = std::min(W, n_steps - k0);
     // frames x(step) .. x(step + w) of this window
-    rc = up2d(s, s->force.p, traj + (size_t)k0 * frame,
-              (size_t)(w + 1) * d.n_svar * d.n_node);
+    if (!in_place)
+      rc = up2d(s, s->force.p, traj + (size_t)k0 * frame,
+                (size_t)(w + 1) * d.n_svar * d.n_node);
     if (rc != TVBX_OK) break;
+    const float *fr = in_place ? traj + (size_t)k0 * frame
+                               : (const float *)s->force.p;
+    s->force_ext = in_place ? fr : nullptr;
     s->force_step0 = s->step;
     s->forced_run = true;
     s->forced_rec = grads;
     rc = tvbx_sim_run(s, w, nullptr, 0);
     s->forced_run = false;
     s->forced_rec = false;
+    s->force_ext = nullptr;
     if (rc != TVBX_OK || !grads) continue;
     tvbx_logp_args L;
     memset(&L, 0, sizeof L);
     fill_args(s, L.f);
-    L.f.force = (const float *)s->force.p;
+    L.f.force = fr;
     L.f.n_steps = (int)w;
     L.rec_sum = (const float *)s->rec_sum.p;
     L.g_c = (double *)s->lg_c.p;
@@ -1808,15 +1831,6 @@ k_rmap_from_avg(const int *__restrict__ rmap, int nvar, int n_inst,
   }
 }
 
-// true when p is device memory that kernels can read and write directly
-static bool on_device(const void *p) {
-  hipPointerAttribute_t at;
-  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
-    (void)hipGetLastError();
-    return false;
-  }
-  return at.type == hipMemoryTypeDevice;
-}
 
 extern "C" int tvbx_rmap_create(tvbx_ctx *c, const int *rmap, int nnode,
                                 int nroi, tvbx_rmap **out) {

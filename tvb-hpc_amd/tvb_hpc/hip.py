@@ -653,6 +653,11 @@ class Simulation:
         check(lib().tvbx_sim_get_outputs(self.h, _ptr(i), _ptr(dr), _ptr(df)))
         return i, dr, df
 
+    def state_into(self, ptr):
+        """Copy the state (S, N, P) float32 to device (or host) memory at
+        address ``ptr`` (e.g. a torch tensor's data_ptr())."""
+        check(lib().tvbx_sim_get_state(self.h, ctypes.c_void_p(ptr)))
+
     def samples_into(self, ptr):
         """Copy the pending monitor samples to device (or host) memory at
         address ``ptr`` ((n, N, P) float32, e.g. a torch tensor's
@@ -703,8 +708,19 @@ class Simulation:
         (logp, {param key: d logp / d param}) for the parameters the
         adjoint differentiates: coupling (P,), model 'ni' (N, P) and
         'inst' (P,)."""
-        tr = np.ascontiguousarray(traj, np.float32)
-        if tr.ndim != 4 or tr.shape[1:] != (self.spec.S, self.N, self.P) \
+        torch = sys.modules.get('torch')
+        if torch is not None and isinstance(traj, torch.Tensor):
+            if not traj.is_cuda or traj.dtype != torch.float32 or \
+                    not traj.is_contiguous():
+                raise ValueError('a torch trajectory must be a contiguous '
+                                 'float32 CUDA tensor')
+            torch.cuda.current_stream(traj.device).synchronize()
+            tr, tptr = traj, ctypes.c_void_p(traj.data_ptr())
+        else:
+            tr = np.ascontiguousarray(traj, np.float32)
+            tptr = _ptr(tr)
+        if tr.ndim != 4 or tuple(tr.shape[1:]) != (self.spec.S, self.N,
+                                                   self.P) \
                 or tr.shape[0] < 2:
             raise ValueError('traj must be (n + 1, %d, %d, %d), n >= 1'
                              % (self.spec.S, self.N, self.P))
@@ -715,7 +731,7 @@ class Simulation:
                              'differentiate')
         g_c = np.zeros((max(len(gc), 1), self.P), np.float32)
         g_m = np.zeros((max(len(gm), 1), self.N, self.P), np.float32)
-        check(lib().tvbx_sim_logp(self.h, _ptr(tr), tr.shape[0] - 1,
+        check(lib().tvbx_sim_logp(self.h, tptr, tr.shape[0] - 1,
                                   int(window), _ptr(out), len(gc), _ptr(g_c),
                                   len(gm), _ptr(g_m)))
         if not grad:
