@@ -234,6 +234,28 @@ def test_temporal_blocking_matches_per_step_launches(model, cfun, dt, cparams,
     assert not simB.flags().any()
 
 
+@pytest.mark.parametrize('model,cfun,dt,cparams', CASES[:4])
+@pytest.mark.parametrize('launch', [
+    dict(ipl=1, waves=4, node_block=4, unroll=4),
+    dict(ipl=4, waves=1, node_block=1, steps_per_launch='L', unroll=4,
+         min_waves=4)])
+def test_pipelined_gathers_match_grouped(model, cfun, dt, cparams, launch):
+    """Issuing the next group of gathers before consuming the current one
+    changes only when loads are issued: bit-identical results."""
+    launch = dict(launch)
+    if launch.get('steps_per_launch') == 'L':
+        launch['steps_per_launch'] = _block_len(dt)
+    P = 256
+    simA, _, noise = _pair(model, cfun, dt, cparams, P=P, steps=400,
+                           launch=launch)
+    simB, _, _ = _pair(model, cfun, dt, cparams, P=P, steps=400,
+                       launch=dict(launch, pipe=True))
+    assert simB.spec.pipe and not simA.spec.pipe
+    simA.run(400, noise=noise)
+    simB.run(400, noise=noise)
+    np.testing.assert_array_equal(simB.get_state(), simA.get_state())
+
+
 @pytest.mark.parametrize('P', [512, 96])
 @pytest.mark.parametrize('blocked', [False, True])
 def test_xcd_block_map_matches_identity(P, blocked):

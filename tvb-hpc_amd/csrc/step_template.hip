@@ -398,6 +398,66 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
 // address words first (the v_readlane results then need no wait states
 // before the loads read them), then every load of the group, then the
 // arithmetic, so that a group's loads are all in flight together.
+#if TVBX_PIPE
+// Pipelined: the next group's loads are issued before the current group's
+// values are consumed, so up to 2 * TVBX_UNROLL gathers are in flight; two
+// register sets (A, B) alternate, with no copies between them.
+#define TVBX_GROUP_ISSUE(V, G0, IX, LD)                                       \
+  {                                                                         \
+    unsigned sx[TVBX_UNROLL];                                               \
+    _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) sx[u] =         \
+        IX((G0) + u);                                                       \
+    __builtin_amdgcn_sched_barrier(0);                                      \
+    _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) V[u] =          \
+        LD(sx[u]);                                                          \
+    __builtin_amdgcn_sched_barrier(0);                                      \
+  }
+#define TVBX_GROUP_USE(V, G0, WW)                                             \
+  _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {                 \
+    const float wu = WW((G0) + u);                                          \
+    float f[TVBX_IPL];                                                      \
+    tvbx_unpack(V[u], f);                                                   \
+    _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc[q] =           \
+        fmaf(wu, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);            \
+  }
+#define TVBX_GATHER_LD(NE, IX, LD, WW)                                        \
+  {                                                                         \
+    int u0 = 0;                                                             \
+    if ((NE) >= TVBX_UNROLL) {                                              \
+      tvbx_raw va[TVBX_UNROLL], vb[TVBX_UNROLL];                            \
+      bool a_pending = true;                                                \
+      TVBX_GROUP_ISSUE(va, 0, IX, LD)                                       \
+      /* every path issues before it consumes, so the wait counts of two */ \
+      /* groups in flight never merge with a path holding fewer */          \
+      while (u0 + 2 * TVBX_UNROLL <= (NE)) {                                \
+        TVBX_GROUP_ISSUE(vb, u0 + TVBX_UNROLL, IX, LD)                      \
+        TVBX_GROUP_USE(va, u0, WW)                                          \
+        u0 += TVBX_UNROLL;                                                  \
+        if (u0 + 2 * TVBX_UNROLL <= (NE)) {                                 \
+          TVBX_GROUP_ISSUE(va, u0 + TVBX_UNROLL, IX, LD)                    \
+          TVBX_GROUP_USE(vb, u0, WW)                                        \
+          u0 += TVBX_UNROLL;                                                \
+        } else {                                                            \
+          TVBX_GROUP_USE(vb, u0, WW)                                        \
+          u0 += TVBX_UNROLL;                                                \
+          a_pending = false;                                                \
+          break;                                                            \
+        }                                                                   \
+      }                                                                     \
+      if (a_pending) {                                                      \
+        TVBX_GROUP_USE(va, u0, WW)                                          \
+        u0 += TVBX_UNROLL;                                                  \
+      }                                                                     \
+    }                                                                       \
+    for (; u0 < (NE); ++u0) {                                               \
+      float f[TVBX_IPL];                                                    \
+      tvbx_unpack(LD(IX(u0)), f);                                           \
+      const float we = WW(u0);                                              \
+      _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc[q] =         \
+          fmaf(we, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);          \
+    }                                                                       \
+  }
+#else
 #define TVBX_GATHER_LD(NE, IX, LD, WW)                                        \
   {                                                                         \
     int u0 = 0;                                                             \
@@ -426,6 +486,7 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
           fmaf(we, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);          \
     }                                                                       \
   }
+#endif
 #endif
 
 // Coupling sums c[q][kk] of node i at step n (ring slot t): the CSR row

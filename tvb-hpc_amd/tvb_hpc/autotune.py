@@ -18,6 +18,8 @@ The choices:
   32): more loads in flight per wave against more registers.
 * ``min_waves``: waves per SIMD the blocked kernel's register budget must
   allow (more resident waves against register spills).
+* ``pipe``: the next group of gathers is issued before the current one is
+  consumed (two groups in flight per wave, two register sets).
 * ``xcd_map``: workgroup numbering that gives each of the 8 XCDs whole
   instance tiles, so the rows that read the same ring line share an L2.
 * temporal blocking: with one node per wave (node_block == waves < N),
@@ -150,7 +152,9 @@ def _time(cfg, cand, x0, steps, warmup, ctx):
 def tune(cfg, steps=100, warmup=10, ctx=None, log=None,
          variants=(dict(unroll=16), dict(unroll=4), dict(min_waves=4),
                    dict(min_waves=4, unroll=4), dict(min_waves=4, unroll=16),
-                   dict(xcd_map=True), dict(xcd_map=True, min_waves=4)),
+                   dict(xcd_map=True), dict(xcd_map=True, min_waves=4),
+                   dict(pipe=True), dict(pipe=True, min_waves=4),
+                   dict(pipe=True, min_waves=4, unroll=4)),
          top=3):
     """Time every launch shape for `steps` steps (and the launch recorded
     before, if any), then the `top` best of each launch width with the
@@ -209,8 +213,9 @@ def batch_candidates(cfg):
     for spl in sorted({1} | {min(dmin + 1, b) for b in (8, 16, 32)}):
         for waves in (1, 2, 4, 8):
             for mw in (1, 4):
-                out.append(dict(ipl=1, waves=waves, steps_per_launch=spl,
-                                min_waves=mw))
+                for pipe in (False, True):
+                    out.append(dict(ipl=1, waves=waves, steps_per_launch=spl,
+                                    min_waves=mw, pipe=pipe))
     return out
 
 

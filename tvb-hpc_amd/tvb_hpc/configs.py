@@ -334,7 +334,8 @@ def all_specs():
         launches += table_launches(cfg)
         for la in launches:
             sp = cfg.spec(ipl=la['ipl'], unroll=la.get('unroll', 8),
-                          min_waves=la.get('min_waves', 1))
+                          min_waves=la.get('min_waves', 1),
+                          pipe=la.get('pipe', False))
             if sp.key() not in keys:
                 keys.add(sp.key())
                 specs.append(sp)
@@ -343,7 +344,8 @@ def all_specs():
         launches = [dict(ipl=1)] + table_launches(cfg)
         for la in launches:
             sp = cfg.spec(ipl=la['ipl'], unroll=la.get('unroll', 8),
-                          min_waves=la.get('min_waves', 1))
+                          min_waves=la.get('min_waves', 1),
+                          pipe=la.get('pipe', False))
             if sp.key() not in keys:
                 keys.add(sp.key())
                 specs.append(sp)

@@ -96,7 +96,7 @@ class KernelSpec:
     def __init__(self, model: BaseModel, cfun: BaseCoupling, scheme: EulerStep,
                  kinds=None, sigma_kind=None, ring='coupled', ring_pre=None,
                  order='cuda', ipl=1, ring_dtype='f32', adjoint=False,
-                 unroll=8, min_waves=1):
+                 unroll=8, min_waves=1, pipe=False):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.adjoint = bool(adjoint)
         if unroll not in (4, 8, 16, 32):
@@ -106,6 +106,9 @@ class KernelSpec:
         if min_waves not in (1, 2, 3, 4, 5, 6, 8):
             raise ValueError('min_waves must be 1 .. 8')
         self.min_waves = int(min_waves)
+        # gathers of the next group issued before the current one is
+        # consumed (two groups in flight)
+        self.pipe = bool(pipe)
         if order not in ('cuda', 'reference'):
             raise ValueError('order must be cuda or reference')
         self.order = order
@@ -263,6 +266,8 @@ class KernelSpec:
         lines.append('#define TVBX_RING_T %d' % RING_TYPES[self.ring_dtype])
         lines.append('#define TVBX_UNROLL %d' % self.unroll)
         lines.append('#define TVBX_MIN_WAVES %d' % self.min_waves)
+        if self.pipe:
+            lines.append('#define TVBX_PIPE 1')
         # lane parameters
         fields = self._lane_fields()
         lines.append('struct tvbx_lane_params {')

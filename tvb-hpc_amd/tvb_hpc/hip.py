@@ -387,7 +387,8 @@ class Simulation:
                  fc=False, sample_slots=64, waves=4, node_block=None,
                  steps_per_launch=1, compat_outputs=False, dt=None,
                  order='cuda', ipl=1, ring_dtype='f32', ring_scale=1.0,
-                 adjoint=False, unroll=8, min_waves=1, xcd_map=False,
+                 adjoint=False, unroll=8, min_waves=1, pipe=False,
+                 xcd_map=False,
                  context=None):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.csr = csr
@@ -411,7 +412,8 @@ class Simulation:
                                     sigma_kind=sigma_kind, ring=ring,
                                     ring_pre=ring_pre, order=order, ipl=ipl,
                                     ring_dtype=ring_dtype, adjoint=adjoint,
-                                    unroll=unroll, min_waves=min_waves)
+                                    unroll=unroll, min_waves=min_waves,
+                                    pipe=pipe)
         if ring_scale != 1.0 and not self.spec.ring_pre:
             raise ValueError('ring_scale needs a ring that holds pre(y)')
         if ring_dtype == 'u8':
@@ -790,7 +792,8 @@ class Batch:
 
     def __init__(self, model, cfun, scheme, subjects, params=None,
                  kinds=None, ring_dtype='f32', ring_scale=1.0, waves=1,
-                 steps_per_launch=1, ipl=1, min_waves=1, unroll=8, dt=None,
+                 steps_per_launch=1, ipl=1, min_waves=1, unroll=8, pipe=False,
+                 dt=None,
                  context=None):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.subjects = [(csr, int(p)) for csr, p in subjects]
@@ -812,7 +815,7 @@ class Batch:
         self.spec = emit.KernelSpec(model, cfun, scheme, kinds=kinds,
                                     sigma_kind=sigma_kind, ipl=ipl,
                                     ring_dtype=ring_dtype, unroll=unroll,
-                                    min_waves=min_waves)
+                                    min_waves=min_waves, pipe=pipe)
         if ring_dtype == 'u8':
             lo, hi = cfun.pre_bounds()
             if lo < 0 or hi * ring_scale > 1.0 + 1e-6:
