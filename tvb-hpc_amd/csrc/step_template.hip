@@ -198,6 +198,60 @@ __device__ __forceinline__ void tvbx_ldr(const tvbx_rt *__restrict__ p,
   tvbx_unpack(tvbx_ldraw(p), o);
 }
 
+// acc[q] += w * pre(value q of a raw ring element group).  An 8-bit history
+// accumulates in units of 2^-24: each byte b is placed in a half word as the
+// float16 subnormal b * 2^-24 (two bytes per v_perm_b32), and v_fma_mix_f32
+// multiplies it by the float32 weight and adds in one fused, once-rounded
+// step.  Scaling by a power of two is exact in every operation here (the
+// partial sums stay far above float32's normal range), so the sums equal the
+// float32 sums of fmaf(w, b, acc) times 2^-24 bit for bit; the caller
+// multiplies the factor 2^24 back into the unscale factor.  With a history
+// holding pre(y) the pre-summation is the identity.
+#define TVBX_MIX_SCALE 16777216.0f
+__device__ __forceinline__ float tvbx_h16(unsigned bits) {
+  return (float)__builtin_bit_cast(_Float16, (unsigned short)bits);
+}
+
+template <typename LP>
+__device__ __forceinline__ void tvbx_accum(float (&acc)[TVBX_IPL], float w,
+                                           const tvbx_raw &v,
+                                           const float (&post)[TVBX_IPL],
+                                           const LP (&lp)[TVBX_IPL],
+                                           const tvbx_step_args &a, int kk) {
+#if TVBX_RING_T == 3
+  (void)post; (void)lp; (void)a; (void)kk;
+#if TVBX_IPL == 4
+  const unsigned h01 = __builtin_amdgcn_perm(0u, (unsigned)v, 0x0c010c00u);
+  const unsigned h23 = __builtin_amdgcn_perm(0u, (unsigned)v, 0x0c030c02u);
+  acc[0] = fmaf(w, tvbx_h16(h01 & 0xffffu), acc[0]);
+  acc[1] = fmaf(w, tvbx_h16(h01 >> 16), acc[1]);
+  acc[2] = fmaf(w, tvbx_h16(h23 & 0xffffu), acc[2]);
+  acc[3] = fmaf(w, tvbx_h16(h23 >> 16), acc[3]);
+#elif TVBX_IPL == 2
+  const unsigned h01 = __builtin_amdgcn_perm(0u, (unsigned)v, 0x0c010c00u);
+  acc[0] = fmaf(w, tvbx_h16(h01 & 0xffffu), acc[0]);
+  acc[1] = fmaf(w, tvbx_h16(h01 >> 16), acc[1]);
+#else
+  acc[0] = fmaf(w, tvbx_h16((unsigned)v), acc[0]);
+#endif
+#else
+  float f[TVBX_IPL];
+  tvbx_unpack(v, f);
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q)
+    acc[q] = fmaf(w, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);
+#endif
+}
+
+// the factor that turns a coupling accumulator into the sum
+__device__ __forceinline__ float tvbx_acc_unscale(const tvbx_step_args &a) {
+#if TVBX_RING_T == 3
+  return a.ring_unscale * TVBX_MIX_SCALE;
+#else
+  return a.ring_unscale;
+#endif
+}
+
 // normals for state variables [s0, s0 + 4) of (instance p, step n, node i)
 __device__ __forceinline__ void tvbx_normals4(const tvbx_step_args &a,
                                               long long n, int i, int p,
@@ -372,22 +426,15 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
                             (int)(w & 0xffffu), lane_off, a);               \
       }                                                                     \
       _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {             \
-        const float wu = WW(u0 + u);                                        \
-        float f[TVBX_IPL];                                                  \
-        tvbx_unpack(v[u], f);                                               \
-        _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc[q] =       \
-            fmaf(wu, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);        \
+        tvbx_accum(acc, WW(u0 + u), v[u], post, lp, a, kk);                 \
       }                                                                     \
     }                                                                       \
     for (; u0 < (NE); ++u0) {                                               \
       const unsigned w = CDW(u0);                                           \
-      float f[TVBX_IPL];                                                    \
-      tvbx_unpack(tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,         \
-                               (int)(w & 0xffffu), lane_off, a),            \
-                  f);                                                       \
-      const float we = WW(u0);                                              \
-      _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc[q] =         \
-          fmaf(we, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);          \
+      tvbx_accum(acc, WW(u0),                                               \
+                 tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,          \
+                              (int)(w & 0xffffu), lane_off, a),             \
+                 post, lp, a, kk);                                          \
     }                                                                       \
   }
 
@@ -414,11 +461,7 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
   }
 #define TVBX_GROUP_USE(V, G0, WW)                                             \
   _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {                 \
-    const float wu = WW((G0) + u);                                          \
-    float f[TVBX_IPL];                                                      \
-    tvbx_unpack(V[u], f);                                                   \
-    _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc[q] =           \
-        fmaf(wu, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);            \
+    tvbx_accum(acc, WW((G0) + u), V[u], post, lp, a, kk);                   \
   }
 #define TVBX_GATHER_LD(NE, IX, LD, WW)                                        \
   {                                                                         \
@@ -450,11 +493,7 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
       }                                                                     \
     }                                                                       \
     for (; u0 < (NE); ++u0) {                                               \
-      float f[TVBX_IPL];                                                    \
-      tvbx_unpack(LD(IX(u0)), f);                                           \
-      const float we = WW(u0);                                              \
-      _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc[q] =         \
-          fmaf(we, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);          \
+      tvbx_accum(acc, WW(u0), LD(IX(u0)), post, lp, a, kk);                 \
     }                                                                       \
   }
 #else
@@ -471,19 +510,11 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
           LD(sx[u]);                                                        \
       __builtin_amdgcn_sched_barrier(0);                                    \
       _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {             \
-        const float wu = WW(u0 + u);                                        \
-        float f[TVBX_IPL];                                                  \
-        tvbx_unpack(v[u], f);                                               \
-        _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc[q] =       \
-            fmaf(wu, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);        \
+        tvbx_accum(acc, WW(u0 + u), v[u], post, lp, a, kk);                 \
       }                                                                     \
     }                                                                       \
     for (; u0 < (NE); ++u0) {                                               \
-      float f[TVBX_IPL];                                                    \
-      tvbx_unpack(LD(IX(u0)), f);                                           \
-      const float we = WW(u0);                                              \
-      _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc[q] =         \
-          fmaf(we, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);          \
+      tvbx_accum(acc, WW(u0), LD(IX(u0)), post, lp, a, kk);                 \
     }                                                                       \
   }
 #endif
@@ -566,14 +597,10 @@ __device__ __forceinline__ void tvbx_coupling(
     // rows longer than 128 edges (rare): one edge at a time, scalar loads
     for (int u0 = 0; u0 < deg - 128; ++u0) {
       const unsigned w = TVBX_CDS(u0);
-      float f[TVBX_IPL];
-      tvbx_unpack(tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,
-                               (int)(w & 0xffffu), lane_off, a),
-                  f);
-      const float we = TVBX_WS(u0);
-#pragma unroll
-      for (int q = 0; q < TVBX_IPL; ++q)
-        acc[q] = fmaf(we, tvbx_pre(kk, f[q], post[q], lp[q], a), acc[q]);
+      tvbx_accum(acc, TVBX_WS(u0),
+                 tvbx_delayed(ring, st_cur, t, (int)(w >> 16), kk,
+                              (int)(w & 0xffffu), lane_off, a),
+                 post, lp, a, kk);
     }
 #undef TVBX_CD0
 #undef TVBX_W0
@@ -583,7 +610,7 @@ __device__ __forceinline__ void tvbx_coupling(
 #undef TVBX_WS
 #pragma unroll
     for (int q = 0; q < TVBX_IPL; ++q) {
-      acc[q] = acc[q] * a.ring_unscale;
+      acc[q] = acc[q] * tvbx_acc_unscale(a);
       c[q][kk] = tvbx_post(kk, acc[q], lp[q], a);
     }
     if (a.rec_sum != nullptr)
