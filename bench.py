@@ -119,7 +119,7 @@ def time_sweep(cfg, launch, K, W, graph_window, profile_steps, fc_idx):
                     'launch_ms': launch_ms, 'launches': launches})
 
 
-def time_gradient(n_evals=2, window=1000):
+def time_gradient(n_evals=2, window=0):
     """Gradient evaluations/s of the FC-fit workload, end to end on the
     device: checkpointed forward of all instances, FC loss against a
     target FC (from a run at G = 0.5) in torch, and the adjoint."""
@@ -157,14 +157,17 @@ def time_gradient(n_evals=2, window=1000):
         del s, loss
     ms = float(np.median(times[1:])) * 1e3
     del sim
-    return {'workload': '%s: %s, %d steps, checkpoint every %d' %
-                        (cfg.name, cfg.description, T, window),
+    return {'workload': '%s: %s, %d steps, %s' %
+                        (cfg.name, cfg.description, T,
+                         'every step recorded in HBM (no replay)'
+                         if window == 0 else
+                         'checkpoint every %d' % window),
             'evals_per_s': 1e3 / ms, 'ms_per_eval': ms,
             'instance_gradients_per_s': P * 1e3 / ms,
             'params': ['c:a (P)', 'm:eta (N x P)'], 'launch': la,
-            'note': 'one evaluation = forward with checkpoints, FC loss in '
-                    'torch on the device, replay + reverse sweep, for all '
-                    'instances'}
+            'note': 'one evaluation = forward (recording x(n) and the '
+                    'coupling sums), FC loss in torch on the device, reverse '
+                    'sweep, for all instances'}
 
 
 def cpu_legs(plain_cfg, fc_idx, n_fc_steps, n_inst, n_steps, threads):

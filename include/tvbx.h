@@ -251,7 +251,11 @@ int tvbx_sim_timing(tvbx_sim *sim, double *ms, long long *launches,
  * no BOLD stage, delays >= 1, additive (or no) noise from Philox.
  *
  * tvbx_sim_checkpoint_run: like tvbx_sim_run for n_steps (a multiple of
- * window), keeping the state and history at every window boundary. */
+ * window), keeping the state and history at every window boundary; the
+ * adjoint replays each window from its checkpoint.  window = 0 instead
+ * records x(n) and the coupling sums of every step during the run
+ * (n_steps * (S + C) * N * pitch * 4 bytes of device memory), and the
+ * adjoint then runs the reverse sweep with no replay. */
 int tvbx_sim_checkpoint_run(tvbx_sim *sim, long long n_steps, int window);
 /* tvbx_sim_adjoint: dL/dsample for samples [sample0, sample0 + n_samples)
  * as [n_samples][N][P] (host, or device memory when on_device) ->

@@ -259,3 +259,25 @@ def test_adjoint_grouped_weights_matches_device_finite_differences():
     assert err < 5e-3
     # the two groups' gradients differ (their weights and delays do)
     assert not np.allclose(g[:P], g[P:][::-1])
+
+
+def test_adjoint_recorded_forward_matches_checkpoints():
+    """window = 0 records every step during the forward run; the reverse
+    sweep then runs without replay and gives the same samples and
+    gradients, bit for bit, as checkpoints and replay."""
+    csr, m, c, sc, G, eta, x0 = _setup()
+    out = []
+    dS = np.random.default_rng(3).standard_normal((T // K, N, P)).astype(
+        np.float32)
+    for window in (WIN, 0):
+        sim = _device_sim(csr, m, c, sc, G, eta)
+        sim.set_state(x0)
+        sim.init_history('zeros')
+        sim.set_noise(SEED)
+        sim.checkpoint_run(T, window)
+        smp, first = sim.samples()
+        out.append((smp, sim.adjoint(dS, first), sim.get_state()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+    for k in out[0][1]:
+        np.testing.assert_array_equal(out[0][1][k], out[1][1][k])

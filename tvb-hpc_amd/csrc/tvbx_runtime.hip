@@ -205,6 +205,10 @@ struct tvbx_sim {
   std::vector<DevBuf> ck_state, ck_ring;
   long long ck_step0 = -1, ck_T = 0;
   int ck_W = 0;
+  // checkpoint_run with window 0: x(n) and sum(n) of every step recorded
+  // during the forward run, so the adjoint needs no replay
+  bool rec_forward = false;
+  bool ck_recorded = false;
   DevBuf lam, mu, rec_x, rec_sum, forcing, g_c, g_m, g_out, save_state,
       save_ring;
 };
@@ -751,6 +755,11 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
   }
   a.ring32 = (double)d.horizon * d.n_ring * s->NP * s->esz < 4294967296.0;
   a.weight_groups = d.weight_groups;
+  if (s->rec_forward) {
+    a.rec_x = (float *)s->rec_x.p;
+    a.rec_sum = (float *)s->rec_sum.p;
+    a.rec_step0 = s->ck_step0;
+  }
   if (s->forced_run) {
     a.force = s->force_ext ? s->force_ext : (const float *)s->force.p;
     a.force_step0 = s->force_step0;
@@ -1355,7 +1364,8 @@ extern "C" int tvbx_sim_run(tvbx_sim *s, long long n_steps,
   const unsigned gx = s->pitch / (64 * d.ipl);
   const unsigned gy = (d.n_node + d.node_block - 1) / d.node_block;
   const unsigned bx = 64u * d.waves;
-  const long long W = (noise || s->forced_run) ? 0 : graph_len(s);
+  const long long W =
+      (noise || s->forced_run || s->rec_forward) ? 0 : graph_len(s);
   long long done = 0;
   while (done < n_steps) {
     if (W > 0 && n_steps - done >= W) {
@@ -2019,12 +2029,31 @@ static int build_csc(tvbx_sim *s) {
 
 extern "C" int tvbx_sim_checkpoint_run(tvbx_sim *s, long long n_steps,
                                        int window) {
-  CHECK(s && n_steps > 0 && window > 0, "bad argument");
-  CHECK(n_steps % window == 0, "n_steps must be a multiple of window");
+  CHECK(s && n_steps > 0 && window >= 0, "bad argument");
+  CHECK(window == 0 || n_steps % window == 0,
+        "n_steps must be a multiple of window");
   int rc = adjoint_checks(s);
   if (rc != TVBX_OK) return rc;
   const tvbx_sim_desc &d = s->d;
   HIPCHK(hipSetDevice(s->ctx->device));
+  s->ck_recorded = false;
+  if (window == 0) {
+    // record the whole run: no checkpoints, no replay in the adjoint
+    CHECK(n_steps <= 0x7fffffffLL, "too many steps to record");
+    rc = dalloc(s->rec_x, (size_t)n_steps * d.n_svar * s->NP * 4);
+    if (rc == TVBX_OK)
+      rc = dalloc(s->rec_sum, (size_t)n_steps * d.n_coupled * s->NP * 4);
+    if (rc != TVBX_OK) return rc;
+    s->ck_step0 = s->step;
+    s->ck_T = n_steps;
+    s->ck_W = (int)n_steps;
+    s->rec_forward = true;
+    rc = tvbx_sim_run(s, n_steps, nullptr, 0);
+    s->rec_forward = false;
+    if (rc != TVBX_OK) return rc;
+    s->ck_recorded = true;
+    return TVBX_OK;
+  }
   const long long K = n_steps / window;
   const size_t sb = s->state.bytes, rb = s->ring.bytes;
   if ((long long)s->ck_state.size() != K) {
@@ -2119,13 +2148,16 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
   const bool blocked = spl > 1 && d.node_block < N;
   const int spl_rev = blocked ? std::max(1, spl - 1) : spl;
   for (long long kw = K - 1; kw >= 0; --kw) {
-    HIPCHK(hipMemcpyAsync(s->state.p, s->ck_state[kw].p, s->state.bytes,
-                          hipMemcpyDeviceToDevice, s->ctx->stream));
-    HIPCHK(hipMemcpyAsync(s->ring.p, s->ck_ring[kw].p, s->ring.bytes,
-                          hipMemcpyDeviceToDevice, s->ctx->stream));
     const long long n_lo = s->ck_step0 + kw * W;
-    // forward replay of the window, recording x(n) and sum(n)
-    for (long long k = 0; k < W; k += spl) {
+    // forward replay of the window, recording x(n) and sum(n), unless the
+    // forward run recorded them
+    if (!s->ck_recorded) {
+      HIPCHK(hipMemcpyAsync(s->state.p, s->ck_state[kw].p, s->state.bytes,
+                            hipMemcpyDeviceToDevice, s->ctx->stream));
+      HIPCHK(hipMemcpyAsync(s->ring.p, s->ck_ring[kw].p, s->ring.bytes,
+                            hipMemcpyDeviceToDevice, s->ctx->stream));
+    }
+    for (long long k = 0; !s->ck_recorded && k < W; k += spl) {
       tvbx_step_args a;
       s->step = n_lo + k;
       fill_args(s, a);

@@ -68,7 +68,9 @@ class _Integrate(torch.autograd.Function):
 def integrate(sim, params, x0, n_steps, window, seed=None):
     """Monitor samples (n_steps // tavg_period, N, P) of ``sim`` from state
     x0 (S, N, P), differentiable w.r.t. the tensors in ``params``
-    ({'c:a': (P,), 'm:eta': (N, P), ...}; keys as in ``Simulation``)."""
+    ({'c:a': (P,), 'm:eta': (N, P), ...}; keys as in ``Simulation``).
+    ``window``: steps between checkpoints, or 0 to record every step in
+    device memory (no forward replay in the backward pass)."""
     keys = list(params)
     return _Integrate.apply(sim, keys, x0, int(n_steps), int(window), seed,
                             *[params[k] for k in keys])

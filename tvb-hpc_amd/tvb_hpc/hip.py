@@ -613,7 +613,9 @@ class Simulation:
 
     def checkpoint_run(self, n_steps, window):
         """Advance n_steps (a multiple of window) keeping checkpoints for
-        ``adjoint`` (requires ``adjoint=True``)."""
+        ``adjoint`` (requires ``adjoint=True``).  window = 0 records every
+        step's state and coupling sums instead ((S + 1) * N * P * 4 bytes
+        per step of device memory): the adjoint then needs no replay."""
         check(lib().tvbx_sim_checkpoint_run(self.h, int(n_steps),
                                             int(window)))
 
