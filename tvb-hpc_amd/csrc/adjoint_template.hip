@@ -20,6 +20,35 @@
 // One reverse step per launch when the node axis is split over workgroups;
 // several (descending, barrier-separated) when one workgroup holds all nodes.
 
+// mu ring row (slot of step n + d, target) that a CSC edge packed as
+// (delay << 16) | target reads at slot t
+__device__ __forceinline__ unsigned tvbx_mu_row(unsigned c, int t, int H,
+                                                int N) {
+  int slot = t + (int)(c >> 16);
+  slot -= slot >= H ? H : 0;
+  return (unsigned)slot * (unsigned)N + (c & 0xffffu);
+}
+
+// TVBX_IPL float32 values through a buffer descriptor
+__device__ __forceinline__ void tvbx_bloadf(__amdgpu_buffer_rsrc_t r,
+                                            unsigned voff, unsigned soff,
+                                            float (&o)[TVBX_IPL]) {
+  typedef float tvbx_f2 __attribute__((ext_vector_type(2)));
+  typedef float tvbx_f4 __attribute__((ext_vector_type(4)));
+#if TVBX_IPL == 4
+  const tvbx_f4 v = __builtin_bit_cast(
+      tvbx_f4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+#elif TVBX_IPL == 2
+  const tvbx_f2 v = __builtin_bit_cast(
+      tvbx_f2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  o[0] = v.x; o[1] = v.y;
+#else
+  o[0] = __builtin_bit_cast(
+      float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+#endif
+}
+
 extern "C" __global__ void __launch_bounds__(1024)
 tvbx_adjoint(const tvbx_adj_args A) {
   const tvbx_step_args &a = A.f;
@@ -51,6 +80,11 @@ tvbx_adjoint(const tvbx_adj_args A) {
   if (one_node && i_begin + wave < i_end)
     tvbx_load_row_of(ccol, crow, cw, i_begin + wave, crw);
   const unsigned lane_off = (unsigned)pb * 4u;
+  // a mu ring under 4 GiB is read through a buffer descriptor with 32-bit
+  // byte offsets computed once per step for the row's first 128 edges
+  const bool mu32 = (unsigned long long)H * NP * 4ull < 4294967296ull;
+  const __amdgpu_buffer_rsrc_t mrs = tvbx_ring_rsrc(mu_r);
+  const unsigned rowb = (unsigned)a.pitch * 4u;
 
   for (int k = 0; k < a.n_steps; ++k) {
     const long long n = a.step0 - k;
@@ -73,6 +107,33 @@ tvbx_adjoint(const tvbx_adj_args A) {
                            ((t + (int)((C) >> 16)) >= H ? H : 0)) * N +       \
                   ((C) & 0xffffu)) * a.pitch) +                               \
       tvbx_lane_offset(lane_off))
+// the same with buffer loads at the byte offsets OV (read back per edge)
+#define TVBX_ADJ_BRANGE(NE, OV, WV)                                           \
+  {                                                                           \
+    int u0 = 0;                                                               \
+    for (; u0 + TVBX_UNROLL <= (NE); u0 += TVBX_UNROLL) {                     \
+      unsigned so[TVBX_UNROLL];                                               \
+      float v[TVBX_UNROLL][TVBX_IPL];                                         \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) so[u] =         \
+          tvbx_lane_u((OV), u0 + u);                                          \
+      __builtin_amdgcn_sched_barrier(0);                                      \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u)                 \
+          tvbx_bloadf(mrs, lane_off, so[u], v[u]);                            \
+      __builtin_amdgcn_sched_barrier(0);                                      \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {               \
+        const float we = tvbx_lane_f((WV), u0 + u);                           \
+        _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) accm[q] =        \
+            fmaf(we, v[u][q], accm[q]);                                       \
+      }                                                                       \
+    }                                                                         \
+    for (; u0 < (NE); ++u0) {                                                 \
+      float v[TVBX_IPL];                                                      \
+      tvbx_bloadf(mrs, lane_off, tvbx_lane_u((OV), u0), v);                   \
+      const float we = tvbx_lane_f((WV), u0);                                 \
+      _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) accm[q] =          \
+          fmaf(we, v[q], accm[q]);                                            \
+    }                                                                         \
+  }
 // TVBX_UNROLL gathers in flight, accumulated in CSC order
 #define TVBX_ADJ_RANGE(NE, CV, WV)                                            \
   {                                                                           \
@@ -98,11 +159,21 @@ tvbx_adjoint(const tvbx_adj_args A) {
           fmaf(we, v[q], accm[q]);                                            \
     }                                                                         \
   }
-      {
+      if (mu32) {
+        // byte offsets of mu(n + d) of the row's targets, this step
+        const unsigned o0 = tvbx_mu_row(crw.cd0, t, H, N) * rowb;
+        const unsigned o1 = tvbx_mu_row(crw.cd1, t, H, N) * rowb;
+        const int n0 = min(deg, 64);
+        TVBX_ADJ_BRANGE(n0, o0, crw.w0)
+        const int n1 = min(max(deg - 64, 0), 64);
+        if (n1 > 0) TVBX_ADJ_BRANGE(n1, o1, crw.w1)
+      } else {
         const int n0 = min(deg, 64);
         TVBX_ADJ_RANGE(n0, crw.cd0, crw.w0)
         const int n1 = min(max(deg - 64, 0), 64);
         if (n1 > 0) TVBX_ADJ_RANGE(n1, crw.cd1, crw.w1)
+      }
+      {
         // rows longer than 128 edges (rare): scalar loads
         for (int e = crw.e0 + 128; e < crw.e1; ++e) {
           const unsigned c = crow[e];
@@ -114,6 +185,7 @@ tvbx_adjoint(const tvbx_adj_args A) {
         }
       }
 #undef TVBX_ADJ_RANGE
+#undef TVBX_ADJ_BRANGE
 #undef TVBX_ADJ_SRC
       float xs[TVBX_S][TVBX_IPL], ls[TVBX_S][TVBX_IPL], sm[TVBX_IPL];
 #pragma unroll
