@@ -86,6 +86,25 @@ tvbx_adjoint(const tvbx_adj_args A) {
   const __amdgpu_buffer_rsrc_t mrs = tvbx_ring_rsrc(mu_r);
   const unsigned rowb = (unsigned)a.pitch * 4u;
 
+  // with one node per wave, lambda and the parameter partials of that node
+  // stay in registers for the whole launch (the same additions, in the same
+  // order, as through memory)
+  const bool keep = one_node && i_begin + wave < i_end;
+  float lr[TVBX_S][TVBX_IPL], gcr[TVBX_NGC][TVBX_IPL], gmr[TVBX_NGM][TVBX_IPL];
+  if (keep) {
+    const size_t jp = (size_t)(i_begin + wave) * a.pitch + pb;
+#pragma unroll
+    for (int s = 0; s < TVBX_S; ++s) tvbx_ldf(A.lam + s * NP + jp, lr[s]);
+#if TVBX_HAS_NGC
+#pragma unroll
+    for (int g = 0; g < TVBX_NGC; ++g) tvbx_ldf(A.g_c + g * NP + jp, gcr[g]);
+#endif
+#if TVBX_HAS_NGM
+#pragma unroll
+    for (int g = 0; g < TVBX_NGM; ++g) tvbx_ldf(A.g_m + g * NP + jp, gmr[g]);
+#endif
+  }
+
   for (int k = 0; k < a.n_steps; ++k) {
     const long long n = a.step0 - k;
     const int t = (int)(n % H);
@@ -191,7 +210,12 @@ tvbx_adjoint(const tvbx_adj_args A) {
 #pragma unroll
       for (int s = 0; s < TVBX_S; ++s) {
         tvbx_ldf(A.rec_x + (w * TVBX_S + s) * NP + jp, xs[s]);
-        tvbx_ldf(A.lam + s * NP + jp, ls[s]);
+        if (keep) {
+#pragma unroll
+          for (int q = 0; q < TVBX_IPL; ++q) ls[s][q] = lr[s][q];
+        } else {
+          tvbx_ldf(A.lam + s * NP + jp, ls[s]);
+        }
       }
       tvbx_ldf(A.rec_sum + w * TVBX_C * NP + jp, sm);
       float mu_o[TVBX_IPL], lo[TVBX_S][TVBX_IPL];
@@ -228,23 +252,51 @@ tvbx_adjoint(const tvbx_adj_args A) {
 #if TVBX_HAS_NGC
 #pragma unroll
         for (int g = 0; g < TVBX_NGC; ++g) {
-          float *d = A.g_c + ((size_t)g * N + j) * a.pitch + p;
-          *d = *d + mu * gc[g];
+          if (keep) {
+            gcr[g][q] = gcr[g][q] + mu * gc[g];
+          } else {
+            float *d = A.g_c + ((size_t)g * N + j) * a.pitch + p;
+            *d = *d + mu * gc[g];
+          }
         }
 #endif
 #if TVBX_HAS_NGM
 #pragma unroll
         for (int g = 0; g < TVBX_NGM; ++g) {
-          float *d = A.g_m + ((size_t)g * N + j) * a.pitch + p;
-          *d = *d + a.dt * gm[g];
+          if (keep) {
+            gmr[g][q] = gmr[g][q] + a.dt * gm[g];
+          } else {
+            float *d = A.g_m + ((size_t)g * N + j) * a.pitch + p;
+            *d = *d + a.dt * gm[g];
+          }
         }
 #endif
       }
+      if (keep) {
 #pragma unroll
-      for (int s = 0; s < TVBX_S; ++s) tvbx_stf(A.lam + s * NP + jp, lo[s]);
+        for (int s = 0; s < TVBX_S; ++s)
+#pragma unroll
+          for (int q = 0; q < TVBX_IPL; ++q) lr[s][q] = lo[s][q];
+      } else {
+#pragma unroll
+        for (int s = 0; s < TVBX_S; ++s) tvbx_stf(A.lam + s * NP + jp, lo[s]);
+      }
       tvbx_stf(A.mu + ((size_t)t * N + j) * a.pitch + pb, mu_o);
     }
     if (k + 1 < a.n_steps) __syncthreads();
+  }
+  if (keep) {
+    const size_t jp = (size_t)(i_begin + wave) * a.pitch + pb;
+#pragma unroll
+    for (int s = 0; s < TVBX_S; ++s) tvbx_stf(A.lam + s * NP + jp, lr[s]);
+#if TVBX_HAS_NGC
+#pragma unroll
+    for (int g = 0; g < TVBX_NGC; ++g) tvbx_stf(A.g_c + g * NP + jp, gcr[g]);
+#endif
+#if TVBX_HAS_NGM
+#pragma unroll
+    for (int g = 0; g < TVBX_NGM; ++g) tvbx_stf(A.g_m + g * NP + jp, gmr[g]);
+#endif
   }
 }
 
