@@ -425,7 +425,6 @@ __host__ __device__ inline size_t tri_index(int i, int j, int N) {
   return (size_t)i * N - (size_t)i * (i - 1) / 2 + (size_t)(j - i);
 }
 
-#define FC_TB 8   // edge of one wave's block of node pairs
 #define FC_ST 8   // tiles per super-tile edge (rows kept in one XCD's L2)
 
 // Merge of a block of m = `count` monitor samples (slots first, first + 1,
@@ -462,103 +461,123 @@ __global__ void k_fc_prep(const float *__restrict__ samples, int slots,
 }
 
 // co-moments of one 16 x 16 tile of node pairs (tile I <= tile J) for the
-// 64 instances of one lane block.  Chunks of FC_CHUNK samples of the tile's
-// 32 rows are staged in LDS by all 256 threads (many independent loads in
-// flight); each of the 4 waves then accumulates an 8 x 8 block of FP64
-// co-moments from LDS (the block strictly below the diagonal of a diagonal
-// tile is skipped).  Workgroups are numbered so that each of the 8 XCDs
-// (which take workgroup ids round robin) works through a contiguous range
-// of (instance block, tile), tiles in super-tile order.
+// 64 instances of one lane block.  Workgroups are numbered so that each of
+// the 8 XCDs (which take workgroup ids round robin) works through a
+// contiguous range of (instance block, tile), tiles in super-tile order.
 #define FC_TILE 16
-#define FC_CHUNK 8
-typedef __attribute__((address_space(3))) void tvbx_lds_void;
-__global__ void __launch_bounds__(256)
+#define FC_KC 4     // samples per MFMA (K of v_mfma_f64_16x16x4f64)
+#define FC_XS ((FC_KC + 1) * 48 + 1)  // LDS dwords per instance (see below)
+typedef double fc_d4 __attribute__((ext_vector_type(4)));
+
+// The block's co-moments C_ij = sum_k d_ki d_kj of one 16 x 16 tile of node
+// pairs for a block of 32 instances, on the FP64 matrix cores: per instance,
+// C += A B with A (16 x 4) the d of the tile's row nodes over 4 samples and
+// B (4 x 16) those of its column nodes; v_mfma_f64_16x16x4f64 takes A[i][k]
+// from lane 16 k + i, B[k][j] from lane 16 k + j, and leaves C[4 r + l / 16]
+// [l % 16] in element r of lane l.  Each of the 4 waves keeps 8 instances'
+// tiles (4 doubles per lane each) for the whole block.
+//   Samples are staged 4 at a time through LDS transposed, instance-major:
+// x[pl][k * 48 + r] (r < 16: row node I0 + r, else column node J0 + r - 16;
+// 16 dwords of padding per sample and one per instance; "sample" 4 holds the
+// 32 nodes' shifts, written once), so the operand reads (lanes = (k, node))
+// fall in distinct banks; wave w stages sample w, its two half-waves the
+// row and the column nodes, and the next 4 samples' global loads are in
+// flight while the current ones are multiplied.  d = x - shift is formed in
+// FP64 at the operand (exact, as the shift is the float running mean).  A
+// small LDS footprint (31 KB) and 128 registers let 4 workgroups share a CU.
+//   The merge into M2 (lanes = instances, coalesced) goes through LDS in 4
+// passes of 64 pairs: in pass m, element m of lane l is pair 64 m + l.
+#define FC_PB 32  // instances per workgroup
+__global__ void __launch_bounds__(256, 4)
 k_fc_update(const float *__restrict__ samples, int slots, long long first,
             int count, int N, int pitch, double n_prev,
             const int2 *__restrict__ tiles, int n_tiles, int n_pblk,
             const float *__restrict__ shift, const double *__restrict__ s1,
             const double *__restrict__ u, double *__restrict__ m2) {
-  __shared__ float xs[FC_CHUNK][2 * FC_TILE][64];
+  __shared__ __attribute__((aligned(16))) float xs[FC_PB * FC_XS];
   const unsigned per = gridDim.x / 8;  // grid is a multiple of 8
   const unsigned L = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (L >= (unsigned)n_tiles * (unsigned)n_pblk) return;  // whole workgroup
   const int pblk = L / n_tiles;
   const int2 tj = tiles[L % n_tiles];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int p = pblk * 64 + lane;
   const int I0 = tj.x * FC_TILE, J0 = tj.y * FC_TILE;
-  const int ra = (wave >> 1) * FC_TB, rb = (wave & 1) * FC_TB;
-  const bool active = !(tj.x == tj.y && ra > rb);
   const size_t NP = (size_t)N * pitch;
-  float shi[FC_TB], shj[FC_TB];
+  const int p0 = pblk * FC_PB;
+  // operand roles of this lane: sample k of the 4, row / column node l % 16
+  const int ok = lane >> 4, on = lane & 15;
+  // staging roles: instance sl of the block, node half sh (rows / columns)
+  const int sl = lane & 31, sh = lane >> 5;
+  const int snode0 = sh ? J0 : I0;
+  // the shifts of the tile's 32 nodes, staged once
 #pragma unroll
-  for (int a = 0; a < FC_TB; ++a) {
-    const int i = I0 + ra + a, j = J0 + rb + a;
-    shi[a] = i < N ? shift[(size_t)i * pitch + p] : 0.0f;
-    shj[a] = j < N ? shift[(size_t)j * pitch + p] : 0.0f;
+  for (int t = 0; t < 4; ++t) {
+    const int r = (2 * wave + sh) * 4 + t;
+    const int node = r < FC_TILE ? I0 + r : J0 + r - FC_TILE;
+    xs[sl * FC_XS + FC_KC * 48 + r] =
+        node < N ? shift[(size_t)node * pitch + p0 + sl] : 0.0f;
   }
-  double c[FC_TB][FC_TB];
+  fc_d4 acc[8];
 #pragma unroll
-  for (int a = 0; a < FC_TB; ++a)
-#pragma unroll
-    for (int b = 0; b < FC_TB; ++b) c[a][b] = 0.0;
+  for (int q = 0; q < 8; ++q) acc[q] = fc_d4{0.0, 0.0, 0.0, 0.0};
   const int slot0 = (int)(first % slots);
-  for (int k0 = 0; k0 < count; k0 += FC_CHUNK) {
-    const int kc = min(FC_CHUNK, count - k0);
-    __syncthreads();  // the previous chunk is consumed
-    // LDS row r: node I0 + r (r < 16) or J0 + r - 16; wave w stages rows
-    // w + 4m straight from global memory into LDS (lane l's dword lands at
-    // the row's LDS base + 4 l), all of the chunk's loads in flight at once
-    for (int kk = 0; kk < kc; ++kk) {
-      int slot = slot0 + k0 + kk;
-      slot -= slot >= slots ? slots : 0;
-      const float *x = samples + (size_t)slot * NP + (size_t)pblk * 64;
+  float pre[16];
+  auto fetch = [&](int k0) {
+    int slot = slot0 + k0 + wave;
+    slot -= slot >= slots ? slots : 0;
+    const bool live = k0 + wave < count;
+    const float *x = samples + (size_t)slot * NP + p0 + sl;
 #pragma unroll
-      for (int m = 0; m < 2 * FC_TILE / 4; ++m) {
-        const int r = wave + 4 * m;
-        const int node = r < FC_TILE ? I0 + r : J0 + r - FC_TILE;
-        if (node < N)
-          __builtin_amdgcn_global_load_lds(x + (size_t)node * pitch + lane,
-                                           (tvbx_lds_void *)&xs[kk][r][0], 4,
-                                           0, 0);
-        else
-          xs[kk][r][lane] = 0.0f;
-      }
+    for (int t = 0; t < 16; ++t) {
+      const int node = snode0 + t;
+      pre[t] = (live && node < N) ? x[(size_t)node * pitch] : 0.0f;
     }
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's loads have landed
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < count; k0 += FC_KC) {
+    __syncthreads();  // the previous samples are consumed
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+      xs[sl * FC_XS + wave * 48 + sh * 16 + t] = pre[t];
     __syncthreads();
-    if (active) {
-      for (int kk = 0; kk < kc; ++kk) {
-        double dj[FC_TB];
+    if (k0 + FC_KC < count) fetch(k0 + FC_KC);  // in flight meanwhile
+    const bool kon = k0 + ok < count;
 #pragma unroll
-        for (int b = 0; b < FC_TB; ++b)
-          dj[b] = (double)xs[kk][FC_TILE + rb + b][lane] - (double)shj[b];
-#pragma unroll
-        for (int a = 0; a < FC_TB; ++a) {
-          const double di = (double)xs[kk][ra + a][lane] - (double)shi[a];
-#pragma unroll
-          for (int b = 0; b < FC_TB; ++b) c[a][b] = fma(di, dj[b], c[a][b]);
-        }
-      }
+    for (int q = 0; q < 8; ++q) {
+      const float *xi = xs + (8 * wave + q) * FC_XS;
+      const float *xq = xi + ok * 48 + on;
+      const float *sq = xi + FC_KC * 48 + on;
+      const double a = kon ? (double)xq[0] - (double)sq[0] : 0.0;
+      const double b = kon ? (double)xq[16] - (double)sq[16] : 0.0;
+      acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
     }
   }
-  if (!active) return;
+  // merge: M2_ij += C_ij - S1_i S1_j / m + u_i u_j n_a m / (n_a + m)
   const double m = (double)count;
-  const double w = n_prev * m / (n_prev + m);
+  const double wgt = n_prev * m / (n_prev + m);
+  double *tb = reinterpret_cast<double *>(xs);  // [64 pairs][33] doubles
+  const bool diag = tj.x == tj.y;
+  const int p = p0 + sl;
 #pragma unroll
-  for (int a = 0; a < FC_TB; ++a) {
-    const int i = I0 + ra + a;
-    if (i >= N) continue;
-    const double s1i = s1[(size_t)i * pitch + p];
-    const double ui = u[(size_t)i * pitch + p];
+  for (int pass = 0; pass < 4; ++pass) {
+    __syncthreads();
 #pragma unroll
-    for (int b = 0; b < FC_TB; ++b) {
-      const int j = J0 + rb + b;
-      if (j >= N || j < i) continue;
+    for (int q = 0; q < 8; ++q) tb[lane * 33 + 8 * wave + q] = acc[q][pass];
+    __syncthreads();
+    // wave w, half sh: pairs 16 w + 2 e + sh of the pass, lanes = instances
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int pl = 16 * wave + 2 * e + sh;
+      const int pi = 64 * pass + pl;
+      const int i = I0 + (pi >> 4), j = J0 + (pi & 15);
+      if (i >= N || j >= N || (diag && j < i)) continue;
+      const double c = tb[pl * 33 + sl];
+      const double s1i = s1[(size_t)i * pitch + p];
       const double s1j = s1[(size_t)j * pitch + p];
+      const double ui = u[(size_t)i * pitch + p];
       const double uj = u[(size_t)j * pitch + p];
       double *dst = m2 + tri_index(i, j, N) * pitch + p;
-      *dst = *dst + (c[a][b] - s1i * s1j / m + ui * uj * w);
+      *dst = *dst + (c - s1i * s1j / m + ui * uj * wgt);
     }
   }
 }
@@ -1201,6 +1220,7 @@ static int fc_flush(tvbx_sim *s) {
   const long long first = (s->sample_base + s->samples_used) % d.sample_slots;
   const double n_prev = (double)s->samples_used;
   const int n_pblk = s->pitch / 64;
+  const int n_fblk = s->pitch / FC_PB;  // instance blocks of the merge
   hipLaunchKernelGGL(k_fc_prep, dim3(n_pblk, N), dim3(64), 0, s->ctx->stream,
                      (const float *)s->samples.p, d.sample_slots, first,
                      (int)pending, N, s->pitch, n_prev,
@@ -1208,12 +1228,12 @@ static int fc_flush(tvbx_sim *s) {
                      (double *)s->fc_s1.p, (double *)s->fc_u.p);
   HIPCHK(hipGetLastError());
   const unsigned long long nwg =
-      ((unsigned long long)n_tiles * n_pblk + 7) / 8 * 8;
+      ((unsigned long long)n_tiles * n_fblk + 7) / 8 * 8;
   CHECK(nwg < (1ull << 31), "FC problem too large for one launch");
   hipLaunchKernelGGL(k_fc_update, dim3((unsigned)nwg), dim3(256), 0,
                      s->ctx->stream, (const float *)s->samples.p,
                      d.sample_slots, first, (int)pending, N, s->pitch, n_prev,
-                     (const int2 *)s->fc_tiles.p, n_tiles, n_pblk,
+                     (const int2 *)s->fc_tiles.p, n_tiles, n_fblk,
                      (const float *)s->fc_shift.p, (const double *)s->fc_s1.p,
                      (const double *)s->fc_u.p, (double *)s->fc_m2.p);
   HIPCHK(hipGetLastError());
