@@ -474,21 +474,21 @@ typedef double fc_d4 __attribute__((ext_vector_type(4)));
 // C += A B with A (16 x 4) the d of the tile's row nodes over 4 samples and
 // B (4 x 16) those of its column nodes; v_mfma_f64_16x16x4f64 takes A[i][k]
 // from lane 16 k + i, B[k][j] from lane 16 k + j, and leaves C[4 r + l / 16]
-// [l % 16] in element r of lane l.  Each of the 4 waves keeps 8 instances'
+// [l % 16] in element r of lane l.  Each of the 8 waves keeps 4 instances'
 // tiles (4 doubles per lane each) for the whole block.
 //   Samples are staged 4 at a time through LDS transposed, instance-major:
 // x[pl][k * 48 + r] (r < 16: row node I0 + r, else column node J0 + r - 16;
 // 16 dwords of padding per sample and one per instance; "sample" 4 holds the
 // 32 nodes' shifts, written once), so the operand reads (lanes = (k, node))
-// fall in distinct banks; wave w stages sample w, its two half-waves the
-// row and the column nodes, and the next 4 samples' global loads are in
-// flight while the current ones are multiplied.  d = x - shift is formed in
-// FP64 at the operand (exact, as the shift is the float running mean).  A
-// small LDS footprint (31 KB) and 128 registers let 4 workgroups share a CU.
+// fall in distinct banks; each half-wave stages 8 node rows of one sample,
+// and the next 4 samples' global loads are in flight while the current ones
+// are multiplied.  d = x - shift is formed in FP64 at the operand (exact, as
+// the shift is the float running mean).  A small LDS footprint (31 KB) and
+// few registers per wave let 3 workgroups of 8 waves share a CU.
 //   The merge into M2 (lanes = instances, coalesced) goes through LDS in 4
 // passes of 64 pairs: in pass m, element m of lane l is pair 64 m + l.
 #define FC_PB 32  // instances per workgroup
-__global__ void __launch_bounds__(256, 4)
+__global__ void __launch_bounds__(512, 6)
 k_fc_update(const float *__restrict__ samples, int slots, long long first,
             int count, int N, int pitch, double n_prev,
             const int2 *__restrict__ tiles, int n_tiles, int n_pblk,
@@ -506,29 +506,32 @@ k_fc_update(const float *__restrict__ samples, int slots, long long first,
   const int p0 = pblk * FC_PB;
   // operand roles of this lane: sample k of the 4, row / column node l % 16
   const int ok = lane >> 4, on = lane & 15;
-  // staging roles: instance sl of the block, node half sh (rows / columns)
+  // staging roles: instance sl of the block; half-wave g = 2 w + sh stages
+  // sample g / 4, node rows 8 (g % 4) .. 8 (g % 4) + 7
   const int sl = lane & 31, sh = lane >> 5;
-  const int snode0 = sh ? J0 : I0;
+  const int sg = 2 * wave + sh;
+  const int sk = sg >> 2, sr0 = (sg & 3) * 8;
+  const int snode0 = sr0 < FC_TILE ? I0 + sr0 : J0 + sr0 - FC_TILE;
   // the shifts of the tile's 32 nodes, staged once
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int r = (2 * wave + sh) * 4 + t;
+  for (int t = 0; t < 2; ++t) {
+    const int r = sg * 2 + t;
     const int node = r < FC_TILE ? I0 + r : J0 + r - FC_TILE;
     xs[sl * FC_XS + FC_KC * 48 + r] =
         node < N ? shift[(size_t)node * pitch + p0 + sl] : 0.0f;
   }
-  fc_d4 acc[8];
+  fc_d4 acc[4];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) acc[q] = fc_d4{0.0, 0.0, 0.0, 0.0};
+  for (int q = 0; q < 4; ++q) acc[q] = fc_d4{0.0, 0.0, 0.0, 0.0};
   const int slot0 = (int)(first % slots);
-  float pre[16];
+  float pre[8];
   auto fetch = [&](int k0) {
-    int slot = slot0 + k0 + wave;
+    int slot = slot0 + k0 + sk;
     slot -= slot >= slots ? slots : 0;
-    const bool live = k0 + wave < count;
-    const float *x = samples + (size_t)slot * NP + p0 + sl;
+    const bool live = k0 + sk < count;
+    const float *x = samples + (size_t)(live ? slot : 0) * NP + p0 + sl;
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
+    for (int t = 0; t < 8; ++t) {
       const int node = snode0 + t;
       pre[t] = (live && node < N) ? x[(size_t)node * pitch] : 0.0f;
     }
@@ -537,14 +540,13 @@ k_fc_update(const float *__restrict__ samples, int slots, long long first,
   for (int k0 = 0; k0 < count; k0 += FC_KC) {
     __syncthreads();  // the previous samples are consumed
 #pragma unroll
-    for (int t = 0; t < 16; ++t)
-      xs[sl * FC_XS + wave * 48 + sh * 16 + t] = pre[t];
+    for (int t = 0; t < 8; ++t) xs[sl * FC_XS + sk * 48 + sr0 + t] = pre[t];
     __syncthreads();
     if (k0 + FC_KC < count) fetch(k0 + FC_KC);  // in flight meanwhile
     const bool kon = k0 + ok < count;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const float *xi = xs + (8 * wave + q) * FC_XS;
+    for (int q = 0; q < 4; ++q) {
+      const float *xi = xs + (4 * wave + q) * FC_XS;
       const float *xq = xi + ok * 48 + on;
       const float *sq = xi + FC_KC * 48 + on;
       const double a = kon ? (double)xq[0] - (double)sq[0] : 0.0;
@@ -562,12 +564,12 @@ k_fc_update(const float *__restrict__ samples, int slots, long long first,
   for (int pass = 0; pass < 4; ++pass) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 8; ++q) tb[lane * 33 + 8 * wave + q] = acc[q][pass];
+    for (int q = 0; q < 4; ++q) tb[lane * 33 + 4 * wave + q] = acc[q][pass];
     __syncthreads();
-    // wave w, half sh: pairs 16 w + 2 e + sh of the pass, lanes = instances
+    // half-wave g: pairs 4 g .. 4 g + 3 of the pass, lanes = instances
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int pl = 16 * wave + 2 * e + sh;
+    for (int e = 0; e < 4; ++e) {
+      const int pl = 4 * sg + e;
       const int pi = 64 * pass + pl;
       const int i = I0 + (pi >> 4), j = J0 + (pi & 15);
       if (i >= N || j >= N || (diag && j < i)) continue;
@@ -1230,7 +1232,7 @@ static int fc_flush(tvbx_sim *s) {
   const unsigned long long nwg =
       ((unsigned long long)n_tiles * n_fblk + 7) / 8 * 8;
   CHECK(nwg < (1ull << 31), "FC problem too large for one launch");
-  hipLaunchKernelGGL(k_fc_update, dim3((unsigned)nwg), dim3(256), 0,
+  hipLaunchKernelGGL(k_fc_update, dim3((unsigned)nwg), dim3(512), 0,
                      s->ctx->stream, (const float *)s->samples.p,
                      d.sample_slots, first, (int)pending, N, s->pitch, n_prev,
                      (const int2 *)s->fc_tiles.p, n_tiles, n_fblk,
