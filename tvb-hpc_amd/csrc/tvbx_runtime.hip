@@ -488,7 +488,11 @@ typedef double fc_d4 __attribute__((ext_vector_type(4)));
 //   The merge into M2 (lanes = instances, coalesced) goes through LDS in 4
 // passes of 64 pairs: in pass m, element m of lane l is pair 64 m + l.
 #define FC_PB 32  // instances per workgroup
-__global__ void __launch_bounds__(512, 6)
+#define FC_NW 8   // waves per workgroup (8 measured faster than 4 or 16)
+#define FC_IW (FC_PB / FC_NW)     // instances per wave
+#define FC_RG (64 / FC_NW)        // node rows staged per half-wave
+#define FC_GS (FC_NW / 2)         // half-waves per staged sample
+__global__ void __launch_bounds__(64 * FC_NW, 6)
 k_fc_update(const float *__restrict__ samples, int slots, long long first,
             int count, int N, int pitch, double n_prev,
             const int2 *__restrict__ tiles, int n_tiles, int n_pblk,
@@ -507,31 +511,31 @@ k_fc_update(const float *__restrict__ samples, int slots, long long first,
   // operand roles of this lane: sample k of the 4, row / column node l % 16
   const int ok = lane >> 4, on = lane & 15;
   // staging roles: instance sl of the block; half-wave g = 2 w + sh stages
-  // sample g / 4, node rows 8 (g % 4) .. 8 (g % 4) + 7
+  // FC_RG node rows of sample g / FC_GS
   const int sl = lane & 31, sh = lane >> 5;
   const int sg = 2 * wave + sh;
-  const int sk = sg >> 2, sr0 = (sg & 3) * 8;
+  const int sk = sg / FC_GS, sr0 = (sg % FC_GS) * FC_RG;
   const int snode0 = sr0 < FC_TILE ? I0 + sr0 : J0 + sr0 - FC_TILE;
   // the shifts of the tile's 32 nodes, staged once
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int r = sg * 2 + t;
+  for (int t = 0; t < 16 / FC_NW; ++t) {
+    const int r = sg * (16 / FC_NW) + t;
     const int node = r < FC_TILE ? I0 + r : J0 + r - FC_TILE;
     xs[sl * FC_XS + FC_KC * 48 + r] =
         node < N ? shift[(size_t)node * pitch + p0 + sl] : 0.0f;
   }
-  fc_d4 acc[4];
+  fc_d4 acc[FC_IW];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) acc[q] = fc_d4{0.0, 0.0, 0.0, 0.0};
+  for (int q = 0; q < FC_IW; ++q) acc[q] = fc_d4{0.0, 0.0, 0.0, 0.0};
   const int slot0 = (int)(first % slots);
-  float pre[8];
+  float pre[FC_RG];
   auto fetch = [&](int k0) {
     int slot = slot0 + k0 + sk;
     slot -= slot >= slots ? slots : 0;
     const bool live = k0 + sk < count;
     const float *x = samples + (size_t)(live ? slot : 0) * NP + p0 + sl;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
+    for (int t = 0; t < FC_RG; ++t) {
       const int node = snode0 + t;
       pre[t] = (live && node < N) ? x[(size_t)node * pitch] : 0.0f;
     }
@@ -540,13 +544,14 @@ k_fc_update(const float *__restrict__ samples, int slots, long long first,
   for (int k0 = 0; k0 < count; k0 += FC_KC) {
     __syncthreads();  // the previous samples are consumed
 #pragma unroll
-    for (int t = 0; t < 8; ++t) xs[sl * FC_XS + sk * 48 + sr0 + t] = pre[t];
+    for (int t = 0; t < FC_RG; ++t)
+      xs[sl * FC_XS + sk * 48 + sr0 + t] = pre[t];
     __syncthreads();
     if (k0 + FC_KC < count) fetch(k0 + FC_KC);  // in flight meanwhile
     const bool kon = k0 + ok < count;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float *xi = xs + (4 * wave + q) * FC_XS;
+    for (int q = 0; q < FC_IW; ++q) {
+      const float *xi = xs + (FC_IW * wave + q) * FC_XS;
       const float *xq = xi + ok * 48 + on;
       const float *sq = xi + FC_KC * 48 + on;
       const double a = kon ? (double)xq[0] - (double)sq[0] : 0.0;
@@ -564,12 +569,13 @@ k_fc_update(const float *__restrict__ samples, int slots, long long first,
   for (int pass = 0; pass < 4; ++pass) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) tb[lane * 33 + 4 * wave + q] = acc[q][pass];
+    for (int q = 0; q < FC_IW; ++q)
+      tb[lane * 33 + FC_IW * wave + q] = acc[q][pass];
     __syncthreads();
-    // half-wave g: pairs 4 g .. 4 g + 3 of the pass, lanes = instances
+    // half-wave g: pairs of the pass 32 g / FC_NW .., lanes = instances
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int pl = 4 * sg + e;
+    for (int e = 0; e < 32 / FC_NW; ++e) {
+      const int pl = (32 / FC_NW) * sg + e;
       const int pi = 64 * pass + pl;
       const int i = I0 + (pi >> 4), j = J0 + (pi & 15);
       if (i >= N || j >= N || (diag && j < i)) continue;
@@ -1232,7 +1238,7 @@ static int fc_flush(tvbx_sim *s) {
   const unsigned long long nwg =
       ((unsigned long long)n_tiles * n_fblk + 7) / 8 * 8;
   CHECK(nwg < (1ull << 31), "FC problem too large for one launch");
-  hipLaunchKernelGGL(k_fc_update, dim3((unsigned)nwg), dim3(512), 0,
+  hipLaunchKernelGGL(k_fc_update, dim3((unsigned)nwg), dim3(64 * FC_NW), 0,
                      s->ctx->stream, (const float *)s->samples.p,
                      d.sample_slots, first, (int)pending, N, s->pitch, n_prev,
                      (const int2 *)s->fc_tiles.p, n_tiles, n_fblk,
