@@ -256,6 +256,25 @@ def test_pipelined_gathers_match_grouped(model, cfun, dt, cparams, launch):
     np.testing.assert_array_equal(simB.get_state(), simA.get_state())
 
 
+@pytest.mark.parametrize('model,cfun,dt,cparams', CASES[:4])
+def test_wave_priority_changes_nothing_but_order(model, cfun, dt, cparams):
+    """Progress-ordered wave priorities in temporally blocked launches change
+    only which wave issues first: bit-identical to equal priorities, and to
+    the oracle within the usual bar."""
+    launch = dict(ipl=1, waves=2, node_block=2,
+                  steps_per_launch=_block_len(dt), min_waves=4, pipe=True)
+    simA, ref, noise = _pair(model, cfun, dt, cparams, P=256, steps=400,
+                             launch=dict(launch, prio=False))
+    simB, _, _ = _pair(model, cfun, dt, cparams, P=256, steps=400,
+                       launch=launch)
+    assert simB.spec.prio and not simA.spec.prio
+    simA.run(400, noise=noise)
+    simB.run(400, noise=noise)
+    ref.run(400, noise=noise)
+    np.testing.assert_array_equal(simB.get_state(), simA.get_state())
+    assert state_err(model, simB.get_state(), ref.get_state()) < TOL
+
+
 @pytest.mark.parametrize('P', [512, 96])
 @pytest.mark.parametrize('blocked', [False, True])
 def test_xcd_block_map_matches_identity(P, blocked):

@@ -106,6 +106,7 @@ tvbx_adjoint(const tvbx_adj_args A) {
   }
 
   for (int k = 0; k < a.n_steps; ++k) {
+    if (one_node) tvbx_progress_prio(k, a.n_steps);
     const long long n = a.step0 - k;
     const int t = (int)(n % H);
     const size_t w = (size_t)(n - a.rec_step0);

@@ -12,6 +12,8 @@
 //                    3 8-bit fixed point (value * 255 of a [0, 1] signal)
 //   TVBX_UNROLL      delayed gathers issued together per CSR row group
 //   TVBX_MIN_WAVES   waves per SIMD the blocked kernel's registers allow
+//   TVBX_PIPE        (defined) two gather groups in flight per wave
+//   TVBX_PRIO        1: wave priority falls with a multi-step launch's progress
 //   struct tvbx_lane_params + tvbx_load_lane()   per-instance parameters
 //   tvbx_pre(), tvbx_post(), tvbx_dfun(), tvbx_obsrv(), tvbx_wrap(),
 //   tvbx_ring_value(), tvbx_bold_step()
@@ -779,6 +781,27 @@ __device__ __forceinline__ void tvbx_block(const tvbx_step_args &a, int &bx,
   }
 }
 
+// Wave priority from a launch's progress: 3, 2, 1, 0 over the four quarters
+// of its n steps.  The waves that share a SIMD in a multi-step launch with
+// no barriers start together; with equal priorities the oldest is issued
+// first, runs ahead and ends early, and the last waves finish the launch
+// alone, with nothing to hide their memory latency.  Falling priority issues
+// the lagging waves first, so the waves of a SIMD stay within a quarter of
+// the launch of each other and end together.
+__device__ __forceinline__ void tvbx_progress_prio(int k, int n) {
+#if TVBX_PRIO
+  const int q = (4 * k) / n;
+  if (q == 0)
+    __builtin_amdgcn_s_setprio(3);
+  else if (q == 1)
+    __builtin_amdgcn_s_setprio(2);
+  else if (q == 2)
+    __builtin_amdgcn_s_setprio(1);
+  else
+    __builtin_amdgcn_s_setprio(0);
+#endif
+}
+
 // step and ring slot of the launch's first step (graph replays add the
 // device clock)
 __device__ __forceinline__ void tvbx_start(const tvbx_step_args &a,
@@ -941,6 +964,7 @@ __device__ __forceinline__ void tvbx_blocked_run(const tvbx_step_args &a,
   tvbx_load_row(a, cd, wt, i, rw);
 
   for (int k = 0; k < a.n_steps; ++k) {
+    tvbx_progress_prio(k, a.n_steps);
     const long long n = n0 + k;
     const int t1 = (t + 1 == H) ? 0 : t + 1;
     if (a.rec_x != nullptr) tvbx_record_state(a, ip, n, x);

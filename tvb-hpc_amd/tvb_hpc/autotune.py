@@ -20,6 +20,11 @@ The choices:
   allow (more resident waves against register spills).
 * ``pipe``: the next group of gathers is issued before the current one is
   consumed (two groups in flight per wave, two register sets).
+* ``prio`` (on unless a launch says otherwise): in multi-step launches
+  without barriers the wave priority falls with the launch's progress, so
+  the waves sharing a SIMD end together instead of the last ones running
+  alone.  It helps where the waves are issue-bound (the 8- and 16-bit
+  rings) and can cost a few per cent where they are purely HBM-bound.
 * ``xcd_map``: workgroup numbering that gives each of the 8 XCDs whole
   instance tiles, so the rows that read the same ring line share an L2.
 * temporal blocking: with one node per wave (node_block == waves < N),
@@ -80,11 +85,11 @@ def candidates(cfg):
 
 
 def block_lengths(cfg):
-    "Temporal-blocking lengths the delays allow (up to d_min + 1, <= 32)."
+    "Temporal-blocking lengths the delays allow (up to d_min + 1, <= 64)."
     if cfg.csr.nnz == 0:
         return []
     lmax = int(np.min(cfg.csr.delays)) + 1
-    return sorted({min(lmax, b) for b in (8, 16, 32)} - {1})
+    return sorted({min(lmax, b) for b in (8, 16, 32, 64)} - {1})
 
 
 def default_launch(cfg):
@@ -154,7 +159,9 @@ def tune(cfg, steps=100, warmup=10, ctx=None, log=None,
                    dict(min_waves=4, unroll=4), dict(min_waves=4, unroll=16),
                    dict(xcd_map=True), dict(xcd_map=True, min_waves=4),
                    dict(pipe=True), dict(pipe=True, min_waves=4),
-                   dict(pipe=True, min_waves=4, unroll=4)),
+                   dict(pipe=True, min_waves=4, unroll=4),
+                   dict(prio=False), dict(prio=False, min_waves=4),
+                   dict(prio=False, min_waves=4, unroll=4)),
          top=3):
     """Time every launch shape for `steps` steps (and the launch recorded
     before, if any), then the `top` best of each launch width with the
@@ -210,12 +217,13 @@ def batch_candidates(cfg):
     "Launch shapes of a subject batch: one node per wave, temporal blocks."
     dmin = min(int(np.min(c.delays)) for c, _ in cfg.subjects if c.nnz)
     out = []
-    for spl in sorted({1} | {min(dmin + 1, b) for b in (8, 16, 32)}):
+    for spl in sorted({1} | {min(dmin + 1, b) for b in (8, 16, 32, 64)}):
         for waves in (1, 2, 4, 8):
             for mw in (1, 4):
                 for pipe in (False, True):
                     out.append(dict(ipl=1, waves=waves, steps_per_launch=spl,
                                     min_waves=mw, pipe=pipe))
+    out += [dict(c, prio=False) for c in out if c['pipe']]
     return out
 
 

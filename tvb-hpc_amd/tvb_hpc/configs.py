@@ -335,7 +335,8 @@ def all_specs():
         for la in launches:
             sp = cfg.spec(ipl=la['ipl'], unroll=la.get('unroll', 8),
                           min_waves=la.get('min_waves', 1),
-                          pipe=la.get('pipe', False))
+                          pipe=la.get('pipe', False),
+                          prio=la.get('prio', True))
             if sp.key() not in keys:
                 keys.add(sp.key())
                 specs.append(sp)
@@ -345,7 +346,8 @@ def all_specs():
         for la in launches:
             sp = cfg.spec(ipl=la['ipl'], unroll=la.get('unroll', 8),
                           min_waves=la.get('min_waves', 1),
-                          pipe=la.get('pipe', False))
+                          pipe=la.get('pipe', False),
+                          prio=la.get('prio', True))
             if sp.key() not in keys:
                 keys.add(sp.key())
                 specs.append(sp)

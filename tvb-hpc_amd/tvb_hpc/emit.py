@@ -96,7 +96,7 @@ class KernelSpec:
     def __init__(self, model: BaseModel, cfun: BaseCoupling, scheme: EulerStep,
                  kinds=None, sigma_kind=None, ring='coupled', ring_pre=None,
                  order='cuda', ipl=1, ring_dtype='f32', adjoint=False,
-                 unroll=8, min_waves=1, pipe=False):
+                 unroll=8, min_waves=1, pipe=False, prio=True):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.adjoint = bool(adjoint)
         if unroll not in (4, 8, 16, 32):
@@ -109,6 +109,8 @@ class KernelSpec:
         # gathers of the next group issued before the current one is
         # consumed (two groups in flight)
         self.pipe = bool(pipe)
+        # wave priority falling with a multi-step launch's progress
+        self.prio = bool(prio)
         if order not in ('cuda', 'reference'):
             raise ValueError('order must be cuda or reference')
         self.order = order
@@ -268,6 +270,7 @@ class KernelSpec:
         lines.append('#define TVBX_MIN_WAVES %d' % self.min_waves)
         if self.pipe:
             lines.append('#define TVBX_PIPE 1')
+        lines.append('#define TVBX_PRIO %d' % int(self.prio))
         # lane parameters
         fields = self._lane_fields()
         lines.append('struct tvbx_lane_params {')

@@ -378,7 +378,7 @@ class Simulation:
         their model / coupling defaults as literals.
     monitor: tavg_period (steps), bold (bool), fc (bool), sample_slots.
     launch: ipl, waves, node_block, steps_per_launch, unroll, min_waves,
-        xcd_map (see tvb_hpc.autotune).
+        pipe, prio, xcd_map (see tvb_hpc.autotune).
     """
 
     def __init__(self, model, cfun, scheme, csr: CSR, n_inst, horizon=None,
@@ -388,7 +388,7 @@ class Simulation:
                  steps_per_launch=1, compat_outputs=False, dt=None,
                  order='cuda', ipl=1, ring_dtype='f32', ring_scale=1.0,
                  adjoint=False, unroll=8, min_waves=1, pipe=False,
-                 xcd_map=False,
+                 prio=True, xcd_map=False,
                  context=None):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.csr = csr
@@ -413,7 +413,7 @@ class Simulation:
                                     ring_pre=ring_pre, order=order, ipl=ipl,
                                     ring_dtype=ring_dtype, adjoint=adjoint,
                                     unroll=unroll, min_waves=min_waves,
-                                    pipe=pipe)
+                                    pipe=pipe, prio=prio)
         if ring_scale != 1.0 and not self.spec.ring_pre:
             raise ValueError('ring_scale needs a ring that holds pre(y)')
         if ring_dtype == 'u8':
@@ -787,15 +787,15 @@ class Batch:
         0 .. k-1 (P_total instances in all).
     params: {name: value}; instance parameters as (P_total,) arrays, node
         parameters as a list of (N_k,) arrays, scalars for globals.
-    launch: ipl, waves (= nodes per workgroup), steps_per_launch, min_waves
-        (see tvb_hpc.autotune).
+    launch: ipl, waves (= nodes per workgroup), steps_per_launch, min_waves,
+        unroll, pipe, prio (see tvb_hpc.autotune).
     Philox noise only (``set_noise``); no monitor.
     """
 
     def __init__(self, model, cfun, scheme, subjects, params=None,
                  kinds=None, ring_dtype='f32', ring_scale=1.0, waves=1,
                  steps_per_launch=1, ipl=1, min_waves=1, unroll=8, pipe=False,
-                 dt=None,
+                 prio=True, dt=None,
                  context=None):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.subjects = [(csr, int(p)) for csr, p in subjects]
@@ -817,7 +817,8 @@ class Batch:
         self.spec = emit.KernelSpec(model, cfun, scheme, kinds=kinds,
                                     sigma_kind=sigma_kind, ipl=ipl,
                                     ring_dtype=ring_dtype, unroll=unroll,
-                                    min_waves=min_waves, pipe=pipe)
+                                    min_waves=min_waves, pipe=pipe,
+                                    prio=prio)
         if ring_dtype == 'u8':
             lo, hi = cfun.pre_bounds()
             if lo < 0 or hi * ring_scale > 1.0 + 1e-6:
