@@ -329,6 +329,7 @@ tvbx_logp_grad(const tvbx_logp_args L) {
     for (int g = 0; g < TVBX_NGC; ++g) gc_acc[q][g] = 0.0;
   }
   for (int k = 0; k < a.n_steps; ++k) {
+    tvbx_progress_prio(k, a.n_steps);
     const float *__restrict__ x0 = a.force + (size_t)k * TVBX_S * NP + ip;
     const float *__restrict__ x1 = x0 + (size_t)TVBX_S * NP;
     const float *__restrict__ sm = L.rec_sum + (size_t)k * TVBX_C * NP + ip;
