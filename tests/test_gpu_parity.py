@@ -501,7 +501,11 @@ def test_validation_errors():
         del sim2
 
 
-def test_nonfinite_flag_marks_only_bad_instance():
+@pytest.mark.parametrize('blocked', [False, True])
+def test_nonfinite_flag_marks_only_bad_instance(blocked):
+    """Per-step launches check every step; temporally blocked launches
+    check the state at the end of each launch (a non-finite state stays
+    non-finite under the unforced update)."""
     dt = 0.1
     csr = small_csr(N=12, dt=dt)
     P = 64
@@ -509,9 +513,12 @@ def test_nonfinite_flag_marks_only_bad_instance():
     c = CFUNS['Linear'](m)
     lam = np.full(P, -1.0, np.float32)
     lam[5] = 1e30  # blows up instance 5 only
+    L = min(int(csr.delays.min()) + 1, 8)
+    launch = dict(waves=1, node_block=1, steps_per_launch=L) if blocked \
+        else {}
     sim = Simulation(m, c, SC.EulerStep(dt), csr, P,
                      params={'m:lam': lam, 'c:a': 0.01},
-                     kinds={'m:lam': 'inst'})
+                     kinds={'m:lam': 'inst'}, **launch)
     sim.set_state(np.ones((1, 12, P), np.float32))
     sim.init_history('constant')
     sim.run(50)
@@ -521,11 +528,15 @@ def test_nonfinite_flag_marks_only_bad_instance():
     assert np.isfinite(np.delete(x, 5, axis=2)).all()
 
 
-def test_reference_rww_singularity_is_flagged():
+@pytest.mark.parametrize('blocked', [False, True])
+def test_reference_rww_singularity_is_flagged(blocked):
     """The reference RWW's h = (a x - b) / (1 - exp(-d (a x - b))) is 0/0 when
     a*x rounds to b; the device flags exactly the instances that hit it,
     the others stay finite, and ReducedWongWang has no such point."""
-    sim, ref, _ = _pair('RWW', 'Linear', 0.1, {'a': 0.05}, em=False)
+    launch = dict(waves=1, node_block=1, steps_per_launch=_block_len(0.1)) \
+        if blocked else None
+    sim, ref, _ = _pair('RWW', 'Linear', 0.1, {'a': 0.05}, em=False,
+                        launch=launch)
     sim.run(1000)
     x = sim.get_state()
     f = sim.flags()

@@ -641,14 +641,25 @@ __device__ __forceinline__ void tvbx_coupling(
   }
 }
 
+// non-finite flags of the lane's instances (bit q: instance pb + q)
+__device__ __forceinline__ void tvbx_flag_nonfinite(
+    const tvbx_step_args &a, int pb, const float (&x)[TVBX_IPL][TVBX_S],
+    unsigned &bad) {
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q)
+#pragma unroll
+    for (int s = 0; s < TVBX_S; ++s)
+      bad |= (unsigned)(!isfinite(x[q][s]) && pb + q < a.n_inst) << q;
+}
+
 // dfun (model.py:105-140), Euler / Euler-Maruyama update (scheme.py:82-97),
-// wrap, non-finite flags and the ring values of x(n + 1).  x is updated in
-// place; k is the step's index within the launch.
+// wrap, non-finite flags (every step with `check`) and the ring values of
+// x(n + 1).  x is updated in place; k is the step's index within the launch.
 __device__ __forceinline__ void tvbx_advance(
     const tvbx_step_args &a, int i, int pb, long long n, int k,
     const tvbx_lane_params (&lp)[TVBX_IPL], float (&x)[TVBX_IPL][TVBX_S],
     const float (&c)[TVBX_IPL][TVBX_C], float (&yo)[TVBX_R][TVBX_IPL],
-    unsigned &bad) {
+    unsigned &bad, bool check) {
   const size_t NP = (size_t)a.n_node * a.pitch;
   const size_t ip = (size_t)i * a.pitch + pb;
 #pragma unroll
@@ -708,9 +719,11 @@ __device__ __forceinline__ void tvbx_advance(
     for (int s = 0; s < TVBX_S; ++s) x[q][s] = x[q][s] + a.dt * f[s];
 #endif
     tvbx_wrap(x[q]);
+    if (check) {
 #pragma unroll
-    for (int s = 0; s < TVBX_S; ++s)
-      bad |= (unsigned)(!isfinite(x[q][s]) && p < a.n_inst) << q;
+      for (int s = 0; s < TVBX_S; ++s)
+        bad |= (unsigned)(!isfinite(x[q][s]) && p < a.n_inst) << q;
+    }
     // observables of x(n + 1) for the history
     tvbx_obsrv(x[q], y, lp[q], a, i, p);
 #pragma unroll
@@ -893,7 +906,7 @@ tvbx_step(const tvbx_step_args a) {
       tvbx_coupling(a, cd, wt, rw, ring, st_cur, i, t, pb, n, lp, c);
 
       float yo[TVBX_R][TVBX_IPL];
-      tvbx_advance(a, i, pb, n, k, lp, x, c, yo, bad);
+      tvbx_advance(a, i, pb, n, k, lp, x, c, yo, bad, true);
 #pragma unroll
       for (int s = 0; s < TVBX_S; ++s) {
         float v[TVBX_IPL];
@@ -993,13 +1006,19 @@ __device__ __forceinline__ void tvbx_blocked_run(const tvbx_step_args &a,
     float c[TVBX_IPL][TVBX_C];
     tvbx_coupling(a, cd, wt, rw, ring, nullptr, i, t, pb, n, lp, c);
     float yo[TVBX_R][TVBX_IPL];
-    tvbx_advance(a, i, pb, n, k, lp, x, c, yo, bad);
+    tvbx_advance(a, i, pb, n, k, lp, x, c, yo, bad, a.force != nullptr);
 #pragma unroll
     for (int r = 0; r < TVBX_R; ++r)
       tvbx_str(ring_w + tvbx_ring_off(t1, r, i, a) + pb, yo[r]);
     if (a.tavg_period > 0) tvbx_monitor(a, ip, n, y_mon, acc_t);
     t = t1;
   }
+  // An unforced update keeps a non-finite state non-finite (x itself is a
+  // term of x + dt f + sqrt(dt) g xi, and the wrap subtracts a constant),
+  // so the state at the end of the launch flags every instance that went
+  // non-finite during it; forced steps, which replace the state, were
+  // checked at every step.
+  if (a.force == nullptr) tvbx_flag_nonfinite(a, pb, x, bad);
 #pragma unroll
   for (int s = 0; s < TVBX_S; ++s) {
     float v[TVBX_IPL];
