@@ -147,6 +147,31 @@ def test_emitter_writes_every_builtin_model(M, C, sch):
     assert '#define TVBX_EM %d' % (sch == 'em') in src
 
 
+def test_kernel_options_reach_the_source():
+    """unroll / min_waves / pipe / prio become the template's defines and
+    give distinct code-object keys; progress priorities are on by default."""
+    m = model.JansenRit()
+    c = coupling.Sigmoidal(m)
+    sch = scheme.EulerStep(0.05)
+    base = emit.KernelSpec(m, c, sch)
+    src = base.source()
+    assert '#define TVBX_PRIO 1' in src and 'TVBX_PIPE 1' not in src
+    assert '#define TVBX_UNROLL 8' in src and '#define TVBX_MIN_WAVES 1' in src
+    keys = {base.key()}
+    for kw, text in ((dict(prio=False), '#define TVBX_PRIO 0'),
+                     (dict(pipe=True), '#define TVBX_PIPE 1'),
+                     (dict(unroll=16), '#define TVBX_UNROLL 16'),
+                     (dict(min_waves=4), '#define TVBX_MIN_WAVES 4')):
+        sp = emit.KernelSpec(m, c, sch, **kw)
+        assert text in sp.source()
+        keys.add(sp.key())
+    assert len(keys) == 5
+    with pytest.raises(ValueError):
+        emit.KernelSpec(m, c, sch, unroll=12)
+    with pytest.raises(ValueError):
+        emit.KernelSpec(m, c, sch, min_waves=7)
+
+
 def test_emitter_rejects_undefined_names():
     class Bad(model.Linear):
         drift = 'lam * x + c + undefined_thing',
