@@ -446,11 +446,14 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
 // TVBX_UNROLL gathers are fenced off from the optimiser's scheduler: all
 // address words first (the v_readlane results then need no wait states
 // before the loads read them), then every load of the group, then the
-// arithmetic, so that a group's loads are all in flight together.
-#if TVBX_PIPE
-// Pipelined: the next group's loads are issued before the current group's
-// values are consumed, so up to 2 * TVBX_UNROLL gathers are in flight; two
-// register sets (A, B) alternate, with no copies between them.
+// arithmetic, so that a group's loads are all in flight together.  The
+// row's last group may be partial: it is issued whole (the address words
+// past the row, u < roundup(NE, TVBX_UNROLL) <= 64, are those of edge
+// words 0, i.e. valid rows of the ring) and its slots past NE are added
+// with weight 0, which leaves a finite sum unchanged (w * pre(y) = +-0); a
+// row costs ceil(NE / TVBX_UNROLL) rounds of loads instead of one round
+// per edge of the remainder.  The weight of a slot past NE is read at edge
+// NE - 1 and replaced by 0, so no read leaves the row.
 #define TVBX_GROUP_ISSUE(V, G0, IX, LD)                                       \
   {                                                                         \
     unsigned sx[TVBX_UNROLL];                                               \
@@ -465,6 +468,18 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
   _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {                 \
     tvbx_accum(acc, WW((G0) + u), V[u], post, lp, a, kk);                   \
   }
+#define TVBX_GROUP_USE_LAST(V, G0, WW, NE)                                    \
+  _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {                 \
+    const int e_ = min((G0) + u, (NE) - 1);                                 \
+    const float w_ = (G0) + u < (NE) ? WW(e_) : 0.0f;                       \
+    tvbx_accum(acc, w_, V[u], post, lp, a, kk);                             \
+  }
+#if TVBX_PIPE
+// Pipelined: the next group's loads are issued before the current group's
+// values are consumed, so up to 2 * TVBX_UNROLL gathers are in flight; two
+// register sets (A, B) alternate, with no copies between them.  Every path
+// issues before it consumes, so the wait counts of two groups in flight
+// never merge with a path holding fewer.
 #define TVBX_GATHER_LD(NE, IX, LD, WW)                                        \
   {                                                                         \
     int u0 = 0;                                                             \
@@ -472,8 +487,6 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
       tvbx_raw va[TVBX_UNROLL], vb[TVBX_UNROLL];                            \
       bool a_pending = true;                                                \
       TVBX_GROUP_ISSUE(va, 0, IX, LD)                                       \
-      /* every path issues before it consumes, so the wait counts of two */ \
-      /* groups in flight never merge with a path holding fewer */          \
       while (u0 + 2 * TVBX_UNROLL <= (NE)) {                                \
         TVBX_GROUP_ISSUE(vb, u0 + TVBX_UNROLL, IX, LD)                      \
         TVBX_GROUP_USE(va, u0, WW)                                          \
@@ -494,8 +507,10 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
         u0 += TVBX_UNROLL;                                                  \
       }                                                                     \
     }                                                                       \
-    for (; u0 < (NE); ++u0) {                                               \
-      tvbx_accum(acc, WW(u0), LD(IX(u0)), post, lp, a, kk);                 \
+    if (u0 < (NE)) {                                                        \
+      tvbx_raw vt[TVBX_UNROLL];                                             \
+      TVBX_GROUP_ISSUE(vt, u0, IX, LD)                                      \
+      TVBX_GROUP_USE_LAST(vt, u0, WW, NE)                                   \
     }                                                                       \
   }
 #else
@@ -503,20 +518,14 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
   {                                                                         \
     int u0 = 0;                                                             \
     for (; u0 + TVBX_UNROLL <= (NE); u0 += TVBX_UNROLL) {                   \
-      unsigned sx[TVBX_UNROLL];                                             \
       tvbx_raw v[TVBX_UNROLL];                                              \
-      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) sx[u] =       \
-          IX(u0 + u);                                                       \
-      __builtin_amdgcn_sched_barrier(0);                                    \
-      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) v[u] =        \
-          LD(sx[u]);                                                        \
-      __builtin_amdgcn_sched_barrier(0);                                    \
-      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {             \
-        tvbx_accum(acc, WW(u0 + u), v[u], post, lp, a, kk);                 \
-      }                                                                     \
+      TVBX_GROUP_ISSUE(v, u0, IX, LD)                                       \
+      TVBX_GROUP_USE(v, u0, WW)                                             \
     }                                                                       \
-    for (; u0 < (NE); ++u0) {                                               \
-      tvbx_accum(acc, WW(u0), LD(IX(u0)), post, lp, a, kk);                 \
+    if (u0 < (NE)) {                                                        \
+      tvbx_raw vt[TVBX_UNROLL];                                             \
+      TVBX_GROUP_ISSUE(vt, u0, IX, LD)                                      \
+      TVBX_GROUP_USE_LAST(vt, u0, WW, NE)                                   \
     }                                                                       \
   }
 #endif
