@@ -256,6 +256,27 @@ def test_pipelined_gathers_match_grouped(model, cfun, dt, cparams, launch):
     np.testing.assert_array_equal(simB.get_state(), simA.get_state())
 
 
+@pytest.mark.parametrize('unroll', [16, 32])
+@pytest.mark.parametrize('pipe', [False, True])
+def test_remainder_group_parity(unroll, pipe):
+    """Rows shorter than a gather group (the small test connectome has
+    rows of a few to ~20 edges): every row ends in a partial group, issued
+    whole with weight-0 slots; the sums match the oracle and are
+    bit-identical to 4-edge groups, whose remainders are shorter."""
+    model, cfun, dt, cp = 'JansenRit', 'Sigmoidal', 0.05, {'a': 0.3}
+    L = _block_len(dt)
+    launch = dict(ipl=1, waves=2, node_block=2, steps_per_launch=L,
+                  min_waves=4, unroll=unroll, pipe=pipe)
+    simA, ref, noise = _pair(model, cfun, dt, cp, P=128, steps=300,
+                             launch=launch)
+    simB, _, _ = _pair(model, cfun, dt, cp, P=128, steps=300,
+                       launch=dict(launch, unroll=4))
+    for s in (simA, simB, ref):
+        s.run(300, noise=noise)
+    np.testing.assert_array_equal(simA.get_state(), simB.get_state())
+    assert state_err(model, simA.get_state(), ref.get_state()) < TOL
+
+
 @pytest.mark.parametrize('model,cfun,dt,cparams', CASES[:4])
 def test_wave_priority_changes_nothing_but_order(model, cfun, dt, cparams):
     """Progress-ordered wave priorities in temporally blocked launches change

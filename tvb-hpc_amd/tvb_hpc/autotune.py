@@ -160,6 +160,7 @@ def tune(cfg, steps=100, warmup=10, ctx=None, log=None,
                    dict(xcd_map=True), dict(xcd_map=True, min_waves=4),
                    dict(pipe=True), dict(pipe=True, min_waves=4),
                    dict(pipe=True, min_waves=4, unroll=4),
+                   dict(pipe=True, min_waves=4, unroll=16),
                    dict(prio=False), dict(prio=False, min_waves=4),
                    dict(prio=False, min_waves=4, unroll=4)),
          top=3):
