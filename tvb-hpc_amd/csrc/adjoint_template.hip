@@ -146,12 +146,22 @@ tvbx_adjoint(const tvbx_adj_args A) {
             fmaf(we, v[u][q], accm[q]);                                       \
       }                                                                       \
     }                                                                         \
-    for (; u0 < (NE); ++u0) {                                                 \
-      float v[TVBX_IPL];                                                      \
-      tvbx_bloadf(mrs, lane_off, tvbx_lane_u((OV), u0), v);                   \
-      const float we = tvbx_lane_f((WV), u0);                                 \
-      _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) accm[q] =          \
-          fmaf(we, v[q], accm[q]);                                            \
+    if (u0 < (NE)) {                                                          \
+      /* the remainder as one group: a slot past the row reads the row's */   \
+      /* last edge again, with the weight 0 its lane holds (adds nothing) */  \
+      unsigned so[TVBX_UNROLL];                                               \
+      float v[TVBX_UNROLL][TVBX_IPL];                                         \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) so[u] =         \
+          tvbx_lane_u((OV), min(u0 + u, (NE) - 1));                           \
+      __builtin_amdgcn_sched_barrier(0);                                      \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u)                 \
+          tvbx_bloadf(mrs, lane_off, so[u], v[u]);                            \
+      __builtin_amdgcn_sched_barrier(0);                                      \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {               \
+        const float we = tvbx_lane_f((WV), u0 + u);                           \
+        _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) accm[q] =        \
+            fmaf(we, v[u][q], accm[q]);                                       \
+      }                                                                       \
     }                                                                         \
   }
 // TVBX_UNROLL gathers in flight, accumulated in CSC order

@@ -447,18 +447,28 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
 // address words first (the v_readlane results then need no wait states
 // before the loads read them), then every load of the group, then the
 // arithmetic, so that a group's loads are all in flight together.  The
-// row's last group may be partial: it is issued whole (the address words
-// past the row, u < roundup(NE, TVBX_UNROLL) <= 64, are those of edge
-// words 0, i.e. valid rows of the ring) and its slots past NE are added
-// with weight 0, which leaves a finite sum unchanged (w * pre(y) = +-0); a
-// row costs ceil(NE / TVBX_UNROLL) rounds of loads instead of one round
-// per edge of the remainder.  The weight of a slot past NE is read at edge
-// NE - 1 and replaced by 0, so no read leaves the row.
+// row's last group may be partial: it is issued whole, a slot past NE
+// gathering edge NE - 1 again, and those slots are added with weight 0,
+// which leaves the sum unchanged (w * pre(y) = +-0 for a finite value, and
+// a non-finite one is in the sum already through the edge itself); a row
+// costs ceil(NE / TVBX_UNROLL) rounds of loads instead of one round per
+// edge of the remainder.  No read leaves the row.
 #define TVBX_GROUP_ISSUE(V, G0, IX, LD)                                       \
   {                                                                         \
     unsigned sx[TVBX_UNROLL];                                               \
     _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) sx[u] =         \
         IX((G0) + u);                                                       \
+    __builtin_amdgcn_sched_barrier(0);                                      \
+    _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) V[u] =          \
+        LD(sx[u]);                                                          \
+    __builtin_amdgcn_sched_barrier(0);                                      \
+  }
+// the row's last, partial group: a slot past NE reads edge NE - 1 again
+#define TVBX_GROUP_ISSUE_LAST(V, G0, IX, LD, NE)                              \
+  {                                                                         \
+    unsigned sx[TVBX_UNROLL];                                               \
+    _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) sx[u] =         \
+        IX(min((G0) + u, (NE) - 1));                                        \
     __builtin_amdgcn_sched_barrier(0);                                      \
     _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) V[u] =          \
         LD(sx[u]);                                                          \
@@ -509,7 +519,7 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
     }                                                                       \
     if (u0 < (NE)) {                                                        \
       tvbx_raw vt[TVBX_UNROLL];                                             \
-      TVBX_GROUP_ISSUE(vt, u0, IX, LD)                                      \
+      TVBX_GROUP_ISSUE_LAST(vt, u0, IX, LD, NE)                             \
       TVBX_GROUP_USE_LAST(vt, u0, WW, NE)                                   \
     }                                                                       \
   }
@@ -524,7 +534,7 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
     }                                                                       \
     if (u0 < (NE)) {                                                        \
       tvbx_raw vt[TVBX_UNROLL];                                             \
-      TVBX_GROUP_ISSUE(vt, u0, IX, LD)                                      \
+      TVBX_GROUP_ISSUE_LAST(vt, u0, IX, LD, NE)                             \
       TVBX_GROUP_USE_LAST(vt, u0, WW, NE)                                   \
     }                                                                       \
   }
