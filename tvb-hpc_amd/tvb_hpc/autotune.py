@@ -93,11 +93,21 @@ def block_lengths(cfg):
 
 
 def default_launch(cfg):
-    """Measured defaults: per-step launches with 4-node workgroups, the
-    widest ipl the instance count allows."""
+    """Defaults for a configuration with no measured entry, after what won
+    for the built-in ones: the widest ipl the instance count allows; when the
+    smallest delay allows blocks of 8 steps or more, one node per wave in
+    blocks of up to 32 steps with the 4-waves-per-SIMD register budget (16
+    gathers per group from 2 instances per lane, 8 pipelined below), else
+    per-step launches with 4-node workgroups."""
     ipls = valid_ipls(cfg)
     ipl = 4 if 4 in ipls and cfg.n_inst >= 1024 else max(
         [i for i in ipls if i <= 2] or [1])
+    bl = block_lengths(cfg)
+    if bl and max(bl) >= 8:
+        la = dict(ipl=ipl, waves=1, node_block=1,
+                  steps_per_launch=min(max(bl), 32), min_waves=4)
+        la.update(dict(unroll=16) if ipl >= 2 else dict(unroll=8, pipe=True))
+        return la
     return dict(ipl=ipl, waves=4, node_block=4, steps_per_launch=1)
 
 

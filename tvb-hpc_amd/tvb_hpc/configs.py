@@ -326,12 +326,12 @@ BATCHES = {'rww_uneven': rww_uneven}
 def all_specs():
     """Kernel specs the builtin configurations use, for every launch width
     the autotuner may pick (for ahead-of-time builds)."""
-    from .autotune import table_launches, valid_ipls
+    from .autotune import default_launch, table_launches, valid_ipls
     specs, keys = [], set()
     for fn in ALL.values():
         cfg = fn()
         launches = [dict(ipl=ipl) for ipl in valid_ipls(cfg)]
-        launches += table_launches(cfg)
+        launches += table_launches(cfg) + [default_launch(cfg)]
         for la in launches:
             sp = cfg.spec(ipl=la['ipl'], unroll=la.get('unroll', 8),
                           min_waves=la.get('min_waves', 1),
