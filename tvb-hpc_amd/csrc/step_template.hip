@@ -553,7 +553,8 @@ __device__ __forceinline__ void tvbx_coupling(
   const size_t NP = (size_t)a.n_node * a.pitch;
   const size_t ip = (size_t)i * a.pitch + pb;
   const unsigned lane_off = (unsigned)pb * (unsigned)sizeof(tvbx_rt);
-  const int deg = row.e1 - row.e0;
+  // (wave-uniform: the row bounds are the same in every lane)
+  const int deg = __builtin_amdgcn_readfirstlane(row.e1 - row.e0);
   const int n0 = min(deg, 64);
   const int n1 = min(max(deg - 64, 0), 64);
   const int eb = __builtin_amdgcn_readfirstlane(row.e0 + 128);
