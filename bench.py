@@ -2,45 +2,59 @@
 Benchmark of the delayed-coupling network integrator on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                    [--plain NAME|none] [--no-grad] [--no-cpu]
+                    [--plain NAME|none] [--variants A,B|none] [--no-grad]
+                    [--no-cpu] [--no-fc]
 
 A "step" is one integration step of the whole sweep batch (all P instances):
 delayed CSR coupling + model dfun + Euler(-Maruyama) update + history write
 + tavg monitor + FP64 online FC.  Inputs are resident in HBM before the
-timed region; the timed region is K steps bracketed by stream
-synchronisations.  Prints one JSON line.
+timed region; the timed region is exactly K steps (the launches of the last
+block are shortened to end on K) bracketed by stream synchronisations.
+Every leg runs on its own: a leg that fails leaves its field null and its
+error text under "errors", and the JSON line is still printed.
 
 value        instance*steps/s of the batched mixed-precision sweep at N=998
-             (Jansen-Rit, P=1024, 32 delay bins, 8-bit fixed-point history of
-             the [0, 1]-scaled sigmoid, FP32 state and sums, FP64 FC):
-             P * K / wall time
-variants     the same sweep with a bfloat16 history (and others on request)
+             (Jansen-Rit, P=1024, delays quantised to 32 bins, bfloat16
+             history of the 1/cmax-scaled sigmoid, FP32 state and sums, FP64
+             FC): P * K / wall time
+variants     the same sweep with a float16 history and with an 8-bit
+             fixed-point history
 plain        the same sweep through the plain FP32 kernel (exact delays,
-             float32 history); vs_plain = value / plain value
+             float32 history); vs_plain = value / plain value (wall clock),
+             vs_plain_kernel the same ratio of step-kernel times
 fc_err_vs_fp64
-             max |FC_device - FC_fp64| over the sampled instances, FC of all
-             W + K steps, for the mixed and the plain path, against the
-             float64 oracle run on the host (exact delays)
+             max |FC_device - FC_fp64| over 8 instances spread over the
+             sweep, after an untimed run of the workload's full length
+             (10,000 steps), for the headline, the plain kernel and the
+             variants, against the float64 oracle (exact delays) on the host
+full_run     wall-clock rate of those full-length runs
 grad         gradient evaluations/s of the FC-fit inversion workload (MPR
-             N=84, P=4096, 20000 steps, checkpoints every 1000): checkpointed
-             forward + adjoint reverse sweep for all instances
+             N=84, P=4096, 20000 steps, checkpoints every 1000 steps replayed
+             in the backward pass): forward + FC loss + adjoint reverse sweep
+             for all instances
 roofline     the step kernel against HBM bandwidth: algorithmic bytes per
              launch (1, 2 or 4 B per edge gather + per node write, per instance
              and step, times the steps of the launch) / mean launch duration
-             (HIP events on the simulation's stream); `traffic` is the
-             measured HBM bytes per launch from the committed rocprofv3 PMC
-             summary for this kernel, workload and launch shape
+             (HIP events on the simulation's stream); `compulsory` counts
+             each distinct (source, delay) history value once per step
+             instead of once per edge; `traffic` is the measured HBM bytes per
+             launch from the committed rocprofv3 PMC summary for this kernel,
+             workload and launch shape
 cpu_baseline the float32 CPU oracle (the reference's loop structure: per
              instance, sequential steps, CSR row reduction, OpenMP over
-             instances) timed on a bounded sample of the plain workload
+             instances), built on the host with -O3 -march=native with and
+             without -ffast-math, timed on a bounded sample of the plain
+             workload on all cores of this process's share
 """
 
 import argparse
 import json
 import os
 import re
+import subprocess
 import sys
 import time
+import traceback
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, 'tvb-hpc_amd'))
@@ -73,10 +87,22 @@ def traffic_from_profile(kernel, workload, launch):
     return float(m.group(1))
 
 
-def time_sweep(cfg, launch, K, W, graph_window, profile_steps, fc_idx):
-    """Run W + K steps (K timed), read the FC of instances fc_idx, then time
-    the step launches one by one with events.  Returns a result dict."""
+def distinct_sources(csr):
+    """Distinct (source, delay) pairs per delay group: the history values a
+    step must read at least once per instance (the compulsory gathers)."""
     import numpy as np
+    col = np.asarray(csr.col, np.int64)
+    out = []
+    for g in range(csr.n_group):
+        d = np.asarray(csr.delays[g], np.int64)
+        out.append(int(np.unique(d * (1 << 32) + col).size))
+    return max(out) if out else 0
+
+
+def time_sweep(cfg, launch, K, W, graph_window, profile_steps):
+    """W untimed steps, then K timed steps (wall clock between stream
+    synchronisations), then `profile_steps` eager launches timed one by one
+    with HIP events on the simulation's stream.  Returns a result dict."""
     sim = cfg.simulation(**launch)
     N, P, nnz = cfg.csr.nnode, cfg.n_inst, cfg.csr.nnz
     sim.set_state(cfg.initial_state())
@@ -91,10 +117,8 @@ def time_sweep(cfg, launch, K, W, graph_window, profile_steps, fc_idx):
     sim.run(K)
     sim.sync()
     el = time.perf_counter() - t0
-    assert not sim.flags().any(), 'non-finite state in the benchmark run'
-    fc = None
-    if cfg.fc and len(fc_idx):
-        fc = np.stack([sim.fc(int(p), 1)[0] for p in fc_idx])
+    if sim.flags().any():
+        raise RuntimeError('non-finite state in the timed run')
     # kernel time: direct launches bracketed by events on the simulation's
     # stream (the timed region above replays graph windows)
     sim.set_exec(graph=False, profile=True)
@@ -103,26 +127,63 @@ def time_sweep(cfg, launch, K, W, graph_window, profile_steps, fc_idx):
     sim.sync()
     ms_kern, launches = sim.timing(reset=True)
     esz = {'f32': 4, 'bf16': 2, 'f16': 2, 'u8': 1}[cfg.ring_dtype]
-    bytes_inst_step = esz * (nnz + N * sim.spec.C)
+    C = sim.spec.C
+    bytes_inst_step = esz * (nnz + N * C)
+    comp_inst_step = esz * (distinct_sources(cfg.csr) * C + N * C)
     steps_per_launch = profile_steps / max(launches, 1)
     launch_ms = ms_kern / max(launches, 1)
-    achieved = bytes_inst_step * P * steps_per_launch / (launch_ms * 1e-3) / 1e9
+    per_s = steps_per_launch / (launch_ms * 1e-3)
+    achieved = bytes_inst_step * P * per_s / 1e9
+    comp = comp_inst_step * P * per_s / 1e9
     kernel = step_kernel(sim)
     del sim
-    return dict(value=P * K / el, ms_per_step=el / K * 1e3, fc=fc,
+    return dict(value=P * K / el, ms_per_step=el / K * 1e3, steps=K,
                 roofline={
                     'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
                     'traffic': None, 'kernel': kernel,
                     'bytes_per_instance_step': bytes_inst_step,
+                    'compulsory': {
+                        'bytes_per_instance_step': comp_inst_step,
+                        'achieved': comp, 'frac': comp / HBM_PEAK_GBS,
+                        'what': 'distinct (source, delay) history values '
+                                'read once + one ring write per node'},
                     'steps_per_launch': steps_per_launch,
-                    'launch_ms': launch_ms, 'launches': launches})
+                    'launch_ms': launch_ms, 'launches': launches,
+                    'kernel_ms_per_step': launch_ms / steps_per_launch})
 
 
-def time_gradient(n_evals=2, window=0):
+def fc_run(cfg, launch, n_steps, fc_idx, graph_window):
+    """Untimed full-length run of n_steps from the initial state; returns the
+    FC of instances fc_idx (FP64 co-moments on the device) and the wall
+    time of the steps."""
+    import numpy as np
+    sim = cfg.simulation(**launch)
+    sim.set_state(cfg.initial_state())
+    sim.init_history('zeros')
+    if getattr(cfg.scheme, 'sigma', None) is not None:
+        sim.set_noise(seed=cfg.seed + 1)
+    sim.set_exec(graph=graph_window > 0, window=max(graph_window, 1),
+                 profile=False)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(n_steps)
+    sim.sync()
+    el = time.perf_counter() - t0
+    if sim.flags().any():
+        raise RuntimeError('non-finite state in the full-length run')
+    fc = np.stack([sim.fc(int(p), 1)[0] for p in fc_idx])
+    ns = sim.n_fc_samples
+    del sim
+    return fc, ns, el
+
+
+def time_gradient(n_evals=2, window=1000):
     """Gradient evaluations/s of the FC-fit workload, end to end on the
-    device: checkpointed forward of all instances, FC loss against a
-    target FC (from a run at G = 0.5) in torch, and the adjoint."""
+    device: checkpointed forward of all instances (a checkpoint every
+    `window` steps, replayed in the backward pass; window = 0 records every
+    step instead), FC loss against a target FC (from a run at G = 0.5) in
+    torch, and the adjoint."""
     import numpy as np
     import torch
     from tvb_hpc import autograd as AG
@@ -170,74 +231,91 @@ def time_gradient(n_evals=2, window=0):
                     'sweep, for all instances'}
 
 
-def cpu_legs(plain_cfg, fc_idx, n_fc_steps, n_inst, n_steps, threads):
-    """The oracle on the host: (1) float32 on a bounded sample of the plain
-    workload, timed -- the CPU baseline; (2) float64 over the benchmark's
-    W + K steps for instances fc_idx -- the reference FC."""
+def host_info():
+    """Cores this process may run on, the machine's logical CPUs and the
+    CPU model (from lscpu)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = None
+    try:
+        out = subprocess.run(['lscpu'], stdout=subprocess.PIPE, text=True,
+                             timeout=20).stdout
+        m = re.search(r'^Model name:\s*(.+)$', out, re.M)
+        model = m.group(1).strip() if m else None
+    except Exception:
+        pass
+    omp = os.environ.get('OMP_NUM_THREADS')
+    return {'nproc': avail, 'machine_cpus': os.cpu_count(),
+            'omp_num_threads': int(omp) if omp and omp.isdigit() else None,
+            'cpu_model': model}
+
+
+def cpu_threads(info):
+    """All cores of this process's share: the affinity set, capped by
+    OMP_NUM_THREADS when the machine sets it (a shared host's quota)."""
+    n = info['nproc']
+    if info['omp_num_threads']:
+        n = min(n, info['omp_num_threads'])
+    return max(1, n)
+
+
+def cpu_baseline(plain_cfg, n_inst, n_steps, threads):
+    """The float32 oracle (the reference's loop structure: per instance,
+    sequential steps, CSR row reduction, OpenMP over instances) built on
+    this host with -O3 -march=native, without and with -ffast-math
+    (examples/hcp100.py:65-72 builds its C path with both), timed on a
+    bounded sample of the plain workload on `threads` cores."""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import numpy as np
     import oracle as O
-
-    def oracle(cfg, idx, dtype):
-        csr = cfg.csr
-        # delay group of every sampled instance (one group per instance)
-        gs = cfg.group_size or cfg.n_inst
-        delays = csr.delays[np.asarray(idx) // gs]
-        weight = csr.weight[np.asarray(idx) // gs] if csr.weight_groups \
-            else csr.weight
-        mp, cp = {}, {}
-        for k, v in cfg.params.items():
-            if k == 'sigma':
-                continue
-            if ':' in k:
-                scope, nm = k.split(':', 1)
-            else:
-                scope, nm = ('c' if k in cfg.cfun.param else 'm'), k
-            a = np.asarray(v)
-            val = a[..., idx] if a.ndim >= 1 and a.shape[-1] == cfg.n_inst \
-                else a
-            (mp if scope == 'm' else cp)[nm] = val
-        cp.update({k: v for k, v in cfg.cfun.param.items()
-                   if k not in cp and not hasattr(v, 'name')})
-        sigma = getattr(cfg.scheme, 'sigma', None)
-        if sigma is not None and 'sigma' in cfg.params:
-            sigma = np.asarray(cfg.params['sigma'])[idx]
-        ref = O.OracleSim(type(cfg.model).__name__, type(cfg.cfun).__name__,
-                          csr.nnode, csr.row, csr.col, weight, delays,
-                          len(idx), csr.max_delay + 2, cfg.scheme.dt,
-                          scheme='euler' if sigma is None else 'em',
-                          params=mp, cparams=cp, sigma=sigma, group_size=1,
-                          dtype=dtype, tavg_period=cfg.tavg_period,
-                          bold=cfg.bold, bold_obs=cfg.bold_obs,
-                          n_threads=threads)
-        ref.set_state(cfg.initial_state()[..., idx])
-        ref.init_history('zeros')
-        return ref
-
     seed = None
     if getattr(plain_cfg.scheme, 'sigma', None) is not None:
         seed = plain_cfg.seed + 1
-
     idx = np.arange(n_inst)
-    ref = oracle(plain_cfg, idx, np.float32)
-    ref.run(5, seed=seed)
-    t = time.perf_counter()
-    ref.run(n_steps, seed=seed)
-    el = time.perf_counter() - t
-    base = {'value': n_inst * n_steps / el, 'unit': 'instance*steps/s',
-            'cores': threads, 'kind': 'port',
-            'sample': '%d instances x %d steps of %s on %d threads (float32 '
-                      'oracle, %.1f s)' % (n_inst, n_steps, plain_cfg.name,
-                                           threads, el)}
-    fc64 = None
-    if len(fc_idx):
-        ref = oracle(plain_cfg, np.asarray(fc_idx), np.float64)
+    res = {}
+    for variant in ('native', 'native_fastmath'):
+        lib = O.load_library(O.build_native(fast_math=variant.endswith(
+            'fastmath')))
+        ref = O.for_config(plain_cfg, idx, np.float32,
+                           n_threads=threads, library=lib)
+        ref.run(5, seed=seed)
         t = time.perf_counter()
-        s = ref.run(n_fc_steps, seed=seed)['samples']
-        log('float64 oracle FC reference: %d instances x %d steps, %.1f s' %
-            (len(fc_idx), n_fc_steps, time.perf_counter() - t))
-        fc64 = np.stack([np.corrcoef(s[:, :, k].T) for k in range(len(fc_idx))])
-    return base, fc64
+        ref.run(n_steps, seed=seed)
+        el = time.perf_counter() - t
+        res[variant] = (n_inst * n_steps / el, el)
+    best = max(res, key=lambda k: res[k][0])
+    info = host_info()
+    return {'value': res[best][0], 'unit': 'instance*steps/s',
+            'cores': threads, 'kind': 'port',
+            'sample': '%d instances x %d steps of %s (float32 oracle, '
+                      '-O3 -march=native -fopenmp%s, %d threads, %.1f s)'
+                      % (n_inst, n_steps, plain_cfg.name,
+                         ' -ffast-math' if best.endswith('fastmath') else '',
+                         threads, res[best][1]),
+            'no_fast_math': res['native'][0],
+            'fast_math': res['native_fastmath'][0],
+            'nproc': info['nproc'], 'machine_cpus': info['machine_cpus'],
+            'cpu_model': info['cpu_model']}
+
+
+def fc64_reference(plain_cfg, fc_idx, n_steps, threads):
+    """FC of instances fc_idx from the float64 oracle over n_steps (exact
+    delays), numpy.corrcoef of its monitor samples."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import numpy as np
+    import oracle as O
+    seed = None
+    if getattr(plain_cfg.scheme, 'sigma', None) is not None:
+        seed = plain_cfg.seed + 1
+    ref = O.for_config(plain_cfg, np.asarray(fc_idx), np.float64,
+                       n_threads=threads)
+    t = time.perf_counter()
+    s = ref.run(n_steps, seed=seed)['samples']
+    log('float64 oracle FC reference: %d instances x %d steps, %.1f s' %
+        (len(fc_idx), n_steps, time.perf_counter() - t))
+    return np.stack([np.corrcoef(s[:, :, k].T) for k in range(len(fc_idx))])
 
 
 RMAP_WORKLOADS = {
@@ -385,7 +463,7 @@ def bench_logp(args):
         n_c = min(args.cpu_inst, P)
         idx = np.arange(n_c)
         fr = frames[:, :, :, :n_c].cpu().numpy()
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads(host_info())
         csr = cfg.csr
         o = O.OracleSim('MontbrioPazoRoxin', 'Linear', N, csr.row, csr.col,
                         csr.weight, csr.delays, n_c, csr.max_delay + 2,
@@ -419,11 +497,13 @@ def bench_batch(args):
     la = dict(autotune.best_batch_launch(cfg))
     log('batch', cfg.name, 'subjects', len(cfg.subjects), 'P', cfg.n_inst,
         'launch', la)
+    gw = 0 if args.no_graph else args.graph_window
     b = cfg.batch(**la)
     for k, x in enumerate(cfg.initial_states()):
         b.set_state(k, x)
     b.init_history('zeros')
     b.set_noise(seed=cfg.seed + 1)
+    b.set_exec(graph=gw > 0, window=max(gw, 1))
     b.run(W)
     b.sync()
     t0 = time.perf_counter()
@@ -431,9 +511,10 @@ def bench_batch(args):
     b.sync()
     el = time.perf_counter() - t0
     assert not b.flags().any(), 'non-finite state in the benchmark run'
-    # launch time: back-to-back launches of one block of steps each
+    # launch time: back-to-back eager launches of one block of steps each
     L = la['steps_per_launch']
     n_l = max(1, args.profile_steps // L)
+    b.set_exec(graph=False)
     b.sync()
     t1 = time.perf_counter()
     b.run(n_l * L)
@@ -454,7 +535,8 @@ def bench_batch(args):
                    'subjects': len(cfg.subjects), 'n_inst': cfg.n_inst,
                    'node_counts': Ns, 'nnz_total': sum(
                        c.nnz for c, _ in cfg.subjects),
-                   'dt_ms': cfg.scheme.dt, 'launch': la},
+                   'dt_ms': cfg.scheme.dt, 'launch': la,
+                   'graph_window': gw},
         'roofline': {'bound': 'hbm', 'achieved': achieved,
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS,
@@ -469,7 +551,7 @@ def bench_batch(args):
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, 'oracle'))
         import oracle as O
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads(host_info())
         # float32 oracle on a bounded sample: the first instances of every
         # subject
         t = 0.0
@@ -511,23 +593,48 @@ def launch_of(cfg, arg):
     return dict(autotune.best_launch(cfg))
 
 
+class Legs:
+    """Runs the benchmark's legs one by one; a leg that raises leaves its
+    field null and its error text in ``errors``, and the line still
+    prints."""
+
+    def __init__(self):
+        self.errors = {}
+
+    def __call__(self, name, fn, *a, **kw):
+        try:
+            return fn(*a, **kw)
+        except Exception as e:  # noqa: BLE001 -- reported, not hidden
+            self.errors[name] = '%s: %s' % (type(e).__name__, e)
+            log('leg %s failed:\n%s' % (name, traceback.format_exc()))
+            return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=None)
+    ap.add_argument('--steps', type=int, default=None,
+                    help='timed steps (default: the workload\'s run length)')
     ap.add_argument('--warmup', type=int, default=200)
-    ap.add_argument('--workload', default='jansenrit998_u8')
-    ap.add_argument('--variants', default='jansenrit998_bf16',
+    ap.add_argument('--workload', default='jansenrit998_bf16')
+    ap.add_argument('--variants', default='jansenrit998_f16,jansenrit998_u8',
                     help='other mixed-precision workloads timed alongside '
                          "(comma separated, or 'none')")
     ap.add_argument('--plain', default='jansenrit998',
                     help="plain-kernel workload to compare with, or 'none'")
     ap.add_argument('--no-grad', action='store_true')
+    ap.add_argument('--grad-no-replay', action='store_true',
+                    help='also time gradients with every step recorded')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-fc', action='store_true',
+                    help='skip the full-length FC error leg')
     ap.add_argument('--fc-instances', type=int, default=8,
                     help='instances whose FC is checked against float64')
+    ap.add_argument('--fc-steps', type=int, default=None,
+                    help='length of the FC error leg (default: the '
+                         'workload\'s run length)')
     ap.add_argument('--cpu-inst', type=int, default=64)
-    ap.add_argument('--cpu-steps', type=int, default=2500)
+    ap.add_argument('--cpu-steps', type=int, default=1000)
     ap.add_argument('--graph-window', type=int, default=100,
                     help='steps per captured graph window')
     ap.add_argument('--no-graph', action='store_true')
@@ -566,58 +673,62 @@ def main():
         return bench_rmap(args)
     if args.workload in LOGP_WORKLOADS:
         return bench_logp(args)
+    legs = Legs()
     gw = 0 if args.no_graph else args.graph_window
     cfg = configs.ALL[args.workload]()
     K = args.steps or cfg.n_steps
     W = args.warmup
     N, P, nnz = cfg.csr.nnode, cfg.n_inst, cfg.csr.nnz
-    # FC of a deterministic sweep is compared with float64 (a stochastic
-    # sweep's sample paths separate from the float64 ones)
-    deterministic = getattr(cfg.scheme, 'sigma', None) is None
-    fc_idx = np.linspace(0, P - 1, args.fc_instances).astype(int) \
-        if cfg.fc and deterministic and args.fc_instances > 0 \
-        else np.zeros(0, int)
     launch = launch_of(cfg, args.launch)
     log('workload', cfg.name, 'N', N, 'P', P, 'nnz', nnz, 'launch', launch)
-    main_r = time_sweep(cfg, launch, K, W, gw, args.profile_steps, fc_idx)
-    main_r['roofline']['traffic'] = traffic_from_profile(
-        main_r['roofline']['kernel'], args.workload, launch)
+    dtype = 'f32' if cfg.ring_dtype == 'f32' else \
+        '%s history, f32 state and sums' % cfg.ring_dtype
     out = {
-        'metric': METRIC,
-        'value': main_r['value'],
-        'unit': 'instance*steps/s',
-        'n_gpus': 1,
-        'steps': K,
-        'warmup': W,
-        'ms_per_step': main_r['ms_per_step'],
-        'higher_is_better': True,
-        'scaling': 'weak',
-        'vs_baseline': None,
-        'dtype': 'f32' if cfg.ring_dtype == 'f32' else
-                 'f32 state, %s history' % cfg.ring_dtype,
-        'data': 'synthetic',
+        'metric': METRIC, 'value': None, 'unit': 'instance*steps/s',
+        'n_gpus': 1, 'steps': K, 'warmup': W, 'ms_per_step': None,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': dtype, 'data': 'synthetic',
         'config': {
             'workload': '%s: %s' % (cfg.name, cfg.description),
             'n_node': N, 'n_inst': P, 'nnz': nnz,
             'delay_bins': cfg.delay_bins, 'dt_ms': cfg.scheme.dt,
             'launch': launch, 'graph_window': gw,
+            'timed': 'K steps of the whole sweep (coupling, dfun, update, '
+                     'history write, tavg monitor, FP64 FC merges), inputs '
+                     'resident in HBM, wall clock between stream '
+                     'synchronisations',
         },
-        'roofline': main_r['roofline'],
+        'roofline': None,
     }
-    plain_r = None
+    main_r = legs('main', time_sweep, cfg, launch, K, W, gw,
+                  args.profile_steps)
+    if main_r is not None:
+        main_r['roofline']['traffic'] = traffic_from_profile(
+            main_r['roofline']['kernel'], cfg.name, launch)
+        out['value'] = main_r['value']
+        out['ms_per_step'] = main_r['ms_per_step']
+        out['roofline'] = main_r['roofline']
+    plain_r, pcfg, pl = None, None, None
     if args.plain != 'none' and args.plain != args.workload:
         pcfg = configs.ALL[args.plain]()
         pl = launch_of(pcfg, None)
         log('plain workload', pcfg.name, 'launch', pl)
-        plain_r = time_sweep(pcfg, pl, K, W, gw, args.profile_steps, fc_idx)
-        plain_r['roofline']['traffic'] = traffic_from_profile(
-            plain_r['roofline']['kernel'], args.plain, pl)
-        out['plain'] = {'workload': '%s: %s' % (pcfg.name, pcfg.description),
-                        'value': plain_r['value'],
-                        'ms_per_step': plain_r['ms_per_step'],
-                        'launch': pl, 'roofline': plain_r['roofline']}
-        out['vs_plain'] = main_r['value'] / plain_r['value']
-    variants = {}
+        plain_r = legs('plain', time_sweep, pcfg, pl, K, W, gw,
+                       args.profile_steps)
+        if plain_r is not None:
+            plain_r['roofline']['traffic'] = traffic_from_profile(
+                plain_r['roofline']['kernel'], pcfg.name, pl)
+            out['plain'] = {'workload': '%s: %s' % (pcfg.name,
+                                                    pcfg.description),
+                            'value': plain_r['value'],
+                            'ms_per_step': plain_r['ms_per_step'],
+                            'launch': pl, 'roofline': plain_r['roofline']}
+            if main_r is not None:
+                out['vs_plain'] = main_r['value'] / plain_r['value']
+                out['vs_plain_kernel'] = (
+                    plain_r['roofline']['kernel_ms_per_step'] /
+                    main_r['roofline']['kernel_ms_per_step'])
+    variants, vcfgs = {}, {}
     for vname in ([] if args.variants == 'none' else
                   args.variants.split(',')):
         if vname in (args.workload, args.plain):
@@ -625,39 +736,78 @@ def main():
         vcfg = configs.ALL[vname]()
         vl = launch_of(vcfg, None)
         log('variant', vcfg.name, 'launch', vl)
-        vr = time_sweep(vcfg, vl, K, W, gw, args.profile_steps, fc_idx)
-        variants[vname] = {'workload': '%s: %s' % (vcfg.name,
-                                                   vcfg.description),
-                           'value': vr['value'],
-                           'ms_per_step': vr['ms_per_step'], 'launch': vl,
-                           'dtype': 'f32 state, %s history' % vcfg.ring_dtype,
-                           'roofline_frac': vr['roofline']['frac'],
-                           'fc': vr['fc']}
-    if not args.no_grad:
-        out['grad'] = time_gradient()
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        pname = args.plain if args.plain != 'none' else args.workload
-        pcfg = configs.ALL[pname]()
-        if pcfg.delay_bins:
-            pcfg = configs.ALL['jansenrit998']()
-        base, fc64 = cpu_legs(pcfg, fc_idx, W + K, min(args.cpu_inst, P),
-                              args.cpu_steps, threads)
-        out['cpu_baseline'] = base
-        if fc64 is not None and main_r['fc'] is not None:
-            err = {'instances': [int(p) for p in fc_idx],
-                   'samples': (W + K) // cfg.tavg_period,
-                   'mixed' if cfg.delay_bins else 'main':
-                       float(np.abs(main_r['fc'] - fc64).max())}
-            if plain_r is not None and plain_r['fc'] is not None:
-                err['plain'] = float(np.abs(plain_r['fc'] - fc64).max())
-            for vname, v in variants.items():
-                if v['fc'] is not None:
-                    err[vname] = float(np.abs(v['fc'] - fc64).max())
-            out['fc_err_vs_fp64'] = err
+        vr = legs('variant:' + vname, time_sweep, vcfg, vl, K, W, gw,
+                  args.profile_steps)
+        if vr is None:
+            continue
+        vcfgs[vname] = (vcfg, vl)
+        variants[vname] = {
+            'workload': '%s: %s' % (vcfg.name, vcfg.description),
+            'value': vr['value'], 'ms_per_step': vr['ms_per_step'],
+            'launch': vl, 'dtype': '%s history, f32 state and sums'
+                                   % vcfg.ring_dtype,
+            'roofline_frac': vr['roofline']['frac'],
+            'roofline_frac_compulsory': vr['roofline']['compulsory']['frac'],
+            'kernel_ms_per_step': vr['roofline']['kernel_ms_per_step']}
+        if plain_r is not None:
+            variants[vname]['vs_plain'] = vr['value'] / plain_r['value']
     if variants:
-        out['variants'] = {k: {kk: vv for kk, vv in v.items() if kk != 'fc'}
-                           for k, v in variants.items()}
+        out['variants'] = variants
+
+    # FC after a full-length run (untimed) against the float64 oracle
+    deterministic = getattr(cfg.scheme, 'sigma', None) is None
+    if cfg.fc and deterministic and args.fc_instances > 0 and \
+            not args.no_fc:
+        n_fc = args.fc_steps or cfg.n_steps
+        fc_idx = np.linspace(0, P - 1, args.fc_instances).astype(int)
+        runs = [(cfg.name, cfg, launch)]
+        if pcfg is not None:
+            runs.append(('plain', pcfg, pl))
+        runs += [(k, c, l) for k, (c, l) in vcfgs.items()]
+        fcs = {}
+        full = {}
+        for key, c, la in runs:
+            r = legs('fc:' + key, fc_run, c, la, n_fc, fc_idx, gw)
+            if r is not None:
+                fcs[key] = r[0]
+                full[key] = {'steps': n_fc, 'fc_samples': r[1],
+                             'value': P * n_fc / r[2],
+                             'ms_per_step': r[2] / n_fc * 1e3}
+        ref_cfg = pcfg if pcfg is not None and not pcfg.delay_bins else \
+            configs.ALL['jansenrit998']()
+        threads = cpu_threads(host_info())
+        fc64 = legs('fc:float64', fc64_reference, ref_cfg, fc_idx, n_fc,
+                    threads) if fcs else None
+        if fc64 is not None:
+            err = {'instances': [int(p) for p in fc_idx], 'steps': n_fc,
+                   'samples': n_fc // cfg.tavg_period,
+                   'reference': 'float64 oracle, exact delays, '
+                                'numpy.corrcoef of its samples'}
+            for key, f in fcs.items():
+                err[key] = float(np.abs(f - fc64).max())
+            out['fc_err_vs_fp64'] = err
+        if full:
+            out['full_run'] = full
+
+    if not args.no_grad:
+        g = legs('grad', time_gradient, window=1000)
+        if g is not None:
+            if args.grad_no_replay:
+                g0 = legs('grad:no_replay', time_gradient, window=0)
+                if g0 is not None:
+                    g['variant_no_replay'] = g0
+            out['grad'] = g
+    if not args.no_cpu:
+        pname = args.plain if args.plain != 'none' else args.workload
+        bcfg = configs.ALL[pname]()
+        if bcfg.delay_bins:
+            bcfg = configs.ALL['jansenrit998']()
+        threads = cpu_threads(host_info())
+        out['cpu_baseline'] = legs('cpu_baseline', cpu_baseline, bcfg,
+                                   min(args.cpu_inst, bcfg.n_inst),
+                                   args.cpu_steps, threads)
+    if legs.errors:
+        out['errors'] = legs.errors
     print(json.dumps(out), flush=True)
 
 

@@ -347,6 +347,18 @@ int tvbx_batch_set_step(tvbx_batch *b, long long step);
 int tvbx_batch_get_step(tvbx_batch *b, long long *step);
 int tvbx_batch_set_noise(tvbx_batch *b, int mode, unsigned long long seed);
 int tvbx_batch_init_history(tvbx_batch *b, int mode);
+/* How tvbx_batch_run launches (replaces the reference's per-subject loop,
+ * examples/hcp100.py:75-118, and the CUDA driver's host launch per window,
+ * dsl/dsl_cuda/example/cuda_run.py:124-142):
+ * persistent = 1: a grid of as many workgroups as the device holds at once,
+ *   each claiming work items from a device counter (vector atomics) until
+ *   none are left, instead of one workgroup per item;
+ * use_graph = 1: whole windows of graph_window steps (rounded down to a
+ *   multiple of steps_per_launch) are captured once as a HIP graph and
+ *   replayed; a device clock gives each replay its first step.
+ * Default: 0, 0, 1. */
+int tvbx_batch_set_exec(tvbx_batch *b, int persistent, int use_graph,
+                        int graph_window);
 int tvbx_batch_run(tvbx_batch *b, long long n_steps);
 int tvbx_batch_sync(tvbx_batch *b);
 /* flags [sum P_s]: 1 where an instance's state became non-finite */

@@ -123,6 +123,7 @@ def quantize(x, ring_dtype):
 
 
 _lib = None
+_native = {}
 
 
 def build(force=False):
@@ -131,33 +132,72 @@ def build(force=False):
     return LIB_PATH
 
 
+def _host_key():
+    """Short key of this host's CPU model and flags: a -march=native build is
+    only valid on the kind of CPU it was compiled on."""
+    import hashlib
+    h = hashlib.sha256()
+    try:
+        with open('/proc/cpuinfo') as fd:
+            for line in fd:
+                if line.startswith(('model name', 'flags')):
+                    h.update(line.encode())
+                    if line.startswith('flags'):
+                        break
+    except OSError:
+        pass
+    return h.hexdigest()[:12]
+
+
+def build_native(fast_math=False):
+    """The oracle compiled for this host as the reference compiles its C
+    path (examples/hcp100.py:65-72): -O3 -march=native -fopenmp, with or
+    without -ffast-math.  Used only as the timed CPU baseline, never as a
+    checker (contraction and fast-math change the float32 rounding)."""
+    sub = 'native_%s%s' % (_host_key(), '_fm' if fast_math else '')
+    out = os.path.join(HERE, '_build', sub, 'libtvboracle.so')
+    if not os.path.exists(out):
+        subprocess.check_call(['make', '-s', '-C', HERE, 'native',
+                               'NATIVE_OUT=_build/' + sub,
+                               'NATIVE_FLAGS=' + ('-ffast-math'
+                                                  if fast_math else '')])
+    return out
+
+
+def load_library(path=None):
+    """ctypes handle of an oracle build (default: the checker build)."""
+    path = path or LIB_PATH
+    if path in _native:
+        return _native[path]
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    L.oracle_run_f32.argtypes = [ctypes.POINTER(OracleSpec)]
+    L.oracle_run_f64.argtypes = [ctypes.POINTER(OracleSpec)]
+    L.oracle_spec_size.restype = ctypes.c_int
+    L.oracle_philox4x32_10.argtypes = [ctypes.c_void_p] * 3
+    L.oracle_normals_f32.argtypes = [ctypes.c_longlong, ctypes.c_uint,
+                                     ctypes.c_uint, ctypes.c_void_p]
+    L.oracle_step_normal_f64.restype = ctypes.c_double
+    L.oracle_step_normal_f64.argtypes = [
+        ctypes.c_uint, ctypes.c_uint, ctypes.c_longlong, ctypes.c_int,
+        ctypes.c_int, ctypes.c_int]
+    L.oracle_step_normal_f32.restype = ctypes.c_float
+    L.oracle_step_normal_f32.argtypes = L.oracle_step_normal_f64.argtypes
+    L.oracle_bold_f64.argtypes = [ctypes.c_void_p, ctypes.c_longlong,
+                                  ctypes.c_double, ctypes.c_void_p,
+                                  ctypes.c_void_p]
+    L.oracle_fc_f64.argtypes = [ctypes.c_void_p, ctypes.c_longlong,
+                                ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    assert L.oracle_spec_size() == ctypes.sizeof(OracleSpec)
+    _native[path] = L
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            build()
-        L = ctypes.CDLL(LIB_PATH)
-        L.oracle_run_f32.argtypes = [ctypes.POINTER(OracleSpec)]
-        L.oracle_run_f64.argtypes = [ctypes.POINTER(OracleSpec)]
-        L.oracle_spec_size.restype = ctypes.c_int
-        L.oracle_philox4x32_10.argtypes = [ctypes.c_void_p] * 3
-        L.oracle_normals_f32.argtypes = [ctypes.c_longlong, ctypes.c_uint,
-                                         ctypes.c_uint, ctypes.c_void_p]
-        L.oracle_step_normal_f64.restype = ctypes.c_double
-        L.oracle_step_normal_f64.argtypes = [
-            ctypes.c_uint, ctypes.c_uint, ctypes.c_longlong, ctypes.c_int,
-            ctypes.c_int, ctypes.c_int]
-        L.oracle_step_normal_f32.restype = ctypes.c_float
-        L.oracle_step_normal_f32.argtypes = \
-            L.oracle_step_normal_f64.argtypes
-        L.oracle_bold_f64.argtypes = [ctypes.c_void_p, ctypes.c_longlong,
-                                      ctypes.c_double, ctypes.c_void_p,
-                                      ctypes.c_void_p]
-        L.oracle_fc_f64.argtypes = [ctypes.c_void_p, ctypes.c_longlong,
-                                    ctypes.c_int, ctypes.c_int,
-                                    ctypes.c_void_p]
-        assert L.oracle_spec_size() == ctypes.sizeof(OracleSpec)
-        _lib = L
+        _lib = load_library(LIB_PATH)
     return _lib
 
 
@@ -258,8 +298,10 @@ class OracleSim:
                  group_size=None, R=1, dtype=np.float64, tavg_period=0,
                  bold=False, bold_obs=0, n_threads=0, order='cuda',
                  ring_dtype='f32', ring_pre=False, ring_scale=1.0,
-                 inst_off=0):
+                 inst_off=0, library=None):
         self.model, self.cfun = model, cfun
+        # ctypes handle of the oracle build to run (default: the checker)
+        self._lib = library
         self.order = {'cuda': 0, 'reference': 1}[order]
         # Philox counters of instance p use inst_off + p
         self.inst_off = int(inst_off)
@@ -455,8 +497,9 @@ class OracleSim:
             keep.append(fr)
             sp.force = _ptr(fr)
             sp.logp = _ptr(logp)
-        f = lib().oracle_run_f64 if self.dtype == np.float64 else \
-            lib().oracle_run_f32
+        L = self._lib or lib()
+        f = L.oracle_run_f64 if self.dtype == np.float64 else \
+            L.oracle_run_f32
         rc = f(ctypes.byref(sp))
         if rc != 0:
             raise RuntimeError('oracle_run failed (%d)' % rc)
@@ -467,3 +510,59 @@ class OracleSim:
         if smp is not None:
             out['samples'] = smp[:sp.samples_written]
         return out
+
+
+def for_config(cfg, idx, dtype=np.float32, H=None, n_threads=0,
+               library=None, emulate_ring=True):
+    """OracleSim for instances ``idx`` of a sweep configuration (an object
+    with the attributes of ``tvb_hpc.configs.SweepConfig``): each instance
+    gets its own delay row (and weight row, for grouped weights), its
+    parameters, sigma and initial state; with a contiguous ``idx`` the
+    Philox counters are those of the device's instance numbering.  With
+    ``emulate_ring`` a reduced-precision history of the config is
+    reproduced (float32 oracle only)."""
+    idx = np.asarray(idx, np.int64)
+    csr = cfg.csr
+    gs = cfg.group_size or cfg.n_inst
+    grp = idx // gs if csr.n_group > 1 else np.zeros_like(idx)
+    delays = np.asarray(csr.delays)[grp]
+    weight = np.asarray(csr.weight)[grp] if csr.weight_groups \
+        else csr.weight
+    mp, cp = {}, {}
+    for k, v in cfg.params.items():
+        if k == 'sigma':
+            continue
+        if ':' in k:
+            scope, nm = k.split(':', 1)
+        else:
+            scope, nm = ('c' if k in cfg.cfun.param else 'm'), k
+        a = np.asarray(v)
+        val = a[..., idx] if a.ndim >= 1 and a.shape[-1] == cfg.n_inst \
+            else a
+        (mp if scope == 'm' else cp)[nm] = val
+    cp.update({k: v for k, v in cfg.cfun.param.items()
+               if k not in cp and not hasattr(v, 'name')})
+    mp.update({k: v for k, v in cfg.model.const.items()
+               if k not in mp and k in MODEL_PARAMS.get(
+                   type(cfg.model).__name__, ()) and np.ndim(v) == 0
+               and not hasattr(v, 'name')})
+    sigma = getattr(cfg.scheme, 'sigma', None)
+    if sigma is not None and 'sigma' in cfg.params:
+        sigma = np.asarray(cfg.params['sigma'])[idx]
+    ring = dict(ring_dtype='f32')
+    if emulate_ring and getattr(cfg, 'ring_dtype', 'f32') != 'f32':
+        ring = dict(ring_dtype=cfg.ring_dtype, ring_pre=True,
+                    ring_scale=cfg.ring_scale)
+    contiguous = idx.size and np.all(np.diff(idx) == 1)
+    ref = OracleSim(type(cfg.model).__name__, type(cfg.cfun).__name__,
+                    csr.nnode, csr.row, csr.col, weight, delays, idx.size,
+                    H or csr.max_delay + 2, cfg.scheme.dt,
+                    scheme='euler' if sigma is None else 'em', params=mp,
+                    cparams=cp, sigma=sigma, group_size=1, dtype=dtype,
+                    tavg_period=cfg.tavg_period, bold=cfg.bold,
+                    bold_obs=cfg.bold_obs, n_threads=n_threads,
+                    inst_off=int(idx[0]) if contiguous else 0,
+                    library=library, **ring)
+    ref.set_state(cfg.initial_state()[..., idx])
+    ref.init_history('zeros')
+    return ref

@@ -144,3 +144,25 @@ def test_batch_validation():
               steps_per_launch=dmin + 1)
     with pytest.raises(ValueError):
         b.set_state(0, np.zeros((1, 5, 64), np.float32))
+
+
+@pytest.mark.parametrize('graph', [False, True])
+def test_batch_work_queue_and_graphs_equal_direct_launches(graph):
+    """Persistent work-queue workgroups (items claimed with an atomic
+    counter) and captured graph windows compute exactly what one workgroup
+    per item launched from the host computes, over several launches (the
+    queue counter must come back to zero after each)."""
+    L = _block_len()
+    ref = None
+    for queue, g in ((False, False), (True, graph)):
+        b, subj, prm, x0, m, c = _batch(dict(waves=2, steps_per_launch=L))
+        b.set_noise(seed=5)
+        b.set_exec(graph=g, window=3 * L, queue=queue)
+        b.run(10 * L + 3)
+        got = [b.get_state(k) for k in range(len(subj))]
+        if ref is None:
+            ref = got
+        else:
+            for a, r in zip(got, ref):
+                np.testing.assert_array_equal(a, r)
+        assert not b.flags().any()

@@ -8,8 +8,6 @@ sides differ only in the last-bit rounding of the transcendental functions
 (libm vs. the device math library).
 """
 
-import os
-
 import numpy as np
 import pytest
 
@@ -452,9 +450,6 @@ def test_monitor_bold_samples_match_oracle():
     err = np.abs(s_dev - s_ref)
     print('max |dBOLD| %.2e of max |BOLD| %.2e' % (err.max(),
                                                    np.abs(s_ref).max()))
-    if err.max() >= 4e-6 and os.path.isdir('gpurun_out'):
-        np.savez('gpurun_out/monitor_samples.npz', dev=s_dev, ref=s_ref,
-                 x_dev=sim.get_state(), x_ref=ref.get_state())
     np.testing.assert_array_equal(sim.get_state(), ref.get_state())
     assert err.max() < 4e-6
     assert err.max() < 2e-3 * np.abs(s_ref).max()
@@ -608,3 +603,48 @@ def test_grouped_weights_match_oracle(launch):
     sim.run(600)
     ref.run(600)
     assert scaled_err(sim.get_state(), ref.get_state()) < TOL
+
+
+@pytest.mark.parametrize('ipl', [1, 2, 4])
+@pytest.mark.parametrize('big_ring', [False, True])
+@pytest.mark.parametrize('launch', [dict(waves=4, node_block=4),
+                                    dict(waves=1, node_block=1,
+                                         steps_per_launch=4, pipe=True)])
+def test_long_rows_match_oracle(ipl, big_ring, launch):
+    """Rows of 150-250 edges, past the two 64-edge register rows: the rest
+    is gathered in 64-edge chunks loaded at every step.  big_ring: a
+    horizon of 60,000 slots makes the ring larger than 4 GiB, so gathers take
+    the 64-bit address path (the oracle, with its own short ring, computes
+    the same values)."""
+    N, P, steps, dt = 300, 64 * ipl, 200, 0.1
+    csr = small_csr(N=N, density=0.7, seed=23, dt=dt, lmax=40.0)
+    deg = np.diff(csr.row)
+    assert deg.max() > 192 and deg.min() > 128
+    m = MODELS['ReducedWongWang']()
+    c = CFUNS['Linear'](m)
+    scheme = SC.EulerMaryuyamaStep(dt)
+    la = dict(launch, ipl=ipl)
+    if la.get('steps_per_launch', 1) > 1:
+        assert int(csr.delays.min()) >= la['steps_per_launch'] - 1
+    H = 60000 if big_ring else None
+    sim = Simulation(m, c, scheme, csr, P, horizon=H,
+                     params={'c:a': np.full(P, 0.02, np.float32),
+                             'sigma': np.full(P, 0.01, np.float32)},
+                     kinds={'sigma': 'inst'}, **la)
+    ring_bytes = sim.H * N * sim.pitch * 4
+    assert (ring_bytes >= 4 << 30) == big_ring
+    rng = np.random.default_rng(3)
+    x0 = m.initial_state(N, P, rng)
+    sim.set_state(x0)
+    sim.init_history('zeros')
+    noise = rng.standard_normal((steps, 1, N, P)).astype(np.float32)
+    sim.run(steps, noise=noise)
+    ref = oracle_for('ReducedWongWang', 'Linear', csr, P, csr.max_delay + 2,
+                     dt, 'em', cparams={'a': 0.02}, sigma=0.01)
+    ref.set_state(x0)
+    ref.init_history('zeros')
+    ref.run(steps, noise=noise)
+    err = scaled_err(sim.get_state(), ref.get_state())
+    print('long rows (%d-%d edges) %s big_ring=%s: %.2e'
+          % (deg.min(), deg.max(), la, big_ring, err))
+    assert err < TOL

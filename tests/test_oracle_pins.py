@@ -300,6 +300,58 @@ def test_oracle_dfun_matches_reference_strings(name):
                                atol=1e-14)
 
 
+COUPLING_CASES = [
+    ('Linear', {'a': 0.7, 'b': 0.2}),
+    ('Diff', {'a': -0.45, 'b': 0.15}),
+    ('Sigmoidal', {'cmin': 0.0, 'cmax': 0.9, 'midpoint': 0.3, 'r': 2.5,
+                   'a': 1.3}),
+    ('Kuramoto', {'a': 1.7, 'b': 0.0}),
+]
+
+
+@pytest.mark.parametrize('cfun,cparams', COUPLING_CASES)
+def test_oracle_coupling_matches_reference_strings(cfun, cparams):
+    """The oracle's hand-written pre / post coupling functions against
+    numpy evaluation of the coupling classes' expression strings (copied
+    verbatim from the reference's coupling.py:103-121), summed over CSR
+    rows with mean = sum / nnode (network.py:115-136): one Euler step of
+    the Linear model with lam = 0 and dt = 1, so x1 - x0 is the coupling
+    term itself, from a history filled with the initial observables and
+    random delays."""
+    from harness import CFUNS
+    N, P = 9, 5
+    rng = np.random.default_rng(17)
+    W = rng.uniform(0.1, 1.0, (N, N)) * (rng.uniform(size=(N, N)) < 0.6)
+    np.fill_diagonal(W, 0.0)
+    A = sparse.csr_matrix(W)
+    row, col, w = A.indptr, A.indices, A.data
+    delays = rng.integers(1, 6, size=col.size)
+    sim = O.OracleSim('Linear', cfun, N, row, col, w, delays, P, 8, 1.0,
+                      params={'lam': 0.0}, cparams=cparams)
+    x0 = rng.uniform(-1.5, 1.5, (1, N, P))
+    sim.set_state(x0)
+    sim.init_history('constant')
+    sim.run(1)
+    got = sim.get_state()[0] - x0[0]
+    m = MODELS['Linear']()
+    c = CFUNS[cfun](m)
+    pre = E.as_expr(c.pre_sum_sym[0])
+    post = E.as_expr(c.post_sum_sym[0])
+    want = np.zeros((N, P))
+    for i in range(N):
+        acc = np.zeros(P)
+        for e in range(row[i], row[i + 1]):
+            env = dict(cparams, pre_syn=x0[0, col[e]], post_syn=x0[0, i])
+            acc = acc + w[e] * np.broadcast_to(E.evaluate(pre, env), (P, ))
+        env = dict(cparams, sum=acc, mean=acc / N)
+        want[i] = np.broadcast_to(E.evaluate(post, env), (P, ))
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-13)
+    # a wrong statistic (mean by the row's degree, or no division) fails
+    if cfun == 'Kuramoto':
+        deg = np.diff(row)[:, None]
+        assert not np.allclose(got, want * N / np.maximum(deg, 1))
+
+
 # ------------------------------------------------------------------ BOLD / FC
 
 def test_bold_rest_is_fixed_point():

@@ -203,15 +203,18 @@ tvbx_adjoint(const tvbx_adj_args A) {
         const int n1 = min(max(deg - 64, 0), 64);
         if (n1 > 0) TVBX_ADJ_RANGE(n1, crw.cd1, crw.w1)
       }
-      {
-        // rows longer than 128 edges (rare): scalar loads
-        for (int e = crw.e0 + 128; e < crw.e1; ++e) {
-          const unsigned c = crow[e];
-          float v[TVBX_IPL];
-          tvbx_ldf(TVBX_ADJ_SRC(c), v);
-          const float we = cw[e];
-#pragma unroll
-          for (int q = 0; q < TVBX_IPL; ++q) accm[q] = fmaf(we, v[q], accm[q]);
+      // rows longer than 128 edges: the rest in chunks of 64 edges, one
+      // edge per lane, gathered in groups like the first 128
+      for (int cb = 128; cb < deg; cb += 64) {
+        const int nc = __builtin_amdgcn_readfirstlane(min(deg - cb, 64));
+        const int ec = crw.e0 + cb + lane;
+        const unsigned cdc = lane < nc ? crow[ec] : 0u;
+        const float wc = lane < nc ? cw[ec] : 0.0f;
+        if (mu32) {
+          const unsigned oc = tvbx_mu_row(cdc, t, H, N) * rowb;
+          TVBX_ADJ_BRANGE(nc, oc, wc)
+        } else {
+          TVBX_ADJ_RANGE(nc, cdc, wc)
         }
       }
 #undef TVBX_ADJ_RANGE

@@ -310,7 +310,8 @@ __device__ __forceinline__ tvbx_raw tvbx_delayed(
 // (delay, column) and weight of the row's edges l and 64 + l.  Each edge is
 // read back with v_readlane into scalar registers, so the row's indices
 // cost two vector loads instead of a chain of scalar loads per group of
-// gathers; edges past the first 128 are read with scalar loads.
+// gathers; edges past the first 128 are loaded at every step in chunks of
+// 64, one edge per lane, and gathered in groups like the first 128.
 struct tvbx_row {
   int e0, e1;        // CSR range
   unsigned cd0, cd1; // lane l: edge e0 + l, e0 + 64 + l
@@ -557,8 +558,9 @@ __device__ __forceinline__ void tvbx_coupling(
   const int deg = __builtin_amdgcn_readfirstlane(row.e1 - row.e0);
   const int n0 = min(deg, 64);
   const int n1 = min(max(deg - 64, 0), 64);
+#if TVBX_ORDER_REF
   const int eb = __builtin_amdgcn_readfirstlane(row.e0 + 128);
-#if !TVBX_ORDER_REF
+#else
   // this step's ring rows of the first 128 edges; as byte offsets when the
   // ring is addressed through a buffer descriptor
   const size_t rowb = (size_t)a.pitch * sizeof(tvbx_rt);
@@ -636,7 +638,8 @@ __device__ __forceinline__ void tvbx_coupling(
 #undef TVBX_WK1
     }
 #endif
-    // rows longer than 128 edges (rare): one edge at a time, scalar loads
+#if TVBX_ORDER_REF
+    // rows longer than 128 edges: one edge at a time, scalar loads
     for (int u0 = 0; u0 < deg - 128; ++u0) {
       const unsigned w = TVBX_CDS(u0);
       tvbx_accum(acc, TVBX_WS(u0),
@@ -644,6 +647,37 @@ __device__ __forceinline__ void tvbx_coupling(
                               (int)(w & 0xffffu), lane_off, a),
                  post, lp, a, kk);
     }
+#else
+    // rows longer than 128 edges: the rest in chunks of 64 edges, one edge
+    // per lane (two vector loads per chunk and step), each chunk gathered in
+    // groups like the first 128 edges
+    for (int cb = 128; cb < deg; cb += 64) {
+      const int nc = __builtin_amdgcn_readfirstlane(min(deg - cb, 64));
+      const int lane = threadIdx.x & 63;
+      const int ec = row.e0 + cb + lane;
+      const unsigned cdc = lane < nc ? cd[ec] : 0u;
+      const float wc = lane < nc ? wt[ec] : 0.0f;
+      unsigned roc = tvbx_edge_row(cdc, t, a);
+#define TVBX_XC(x) tvbx_lane_u(roc, (x))
+#define TVBX_WC(x) tvbx_lane_f(wc, (x))
+      if (a.ring32) {
+        roc *= (unsigned)rowb;
+        const unsigned voff =
+            lane_off + (unsigned)(kk * a.n_node) * (unsigned)rowb;
+#define TVBX_LB(o) tvbx_bload(rs, voff, (o))
+        TVBX_GATHER_LD(nc, TVBX_XC, TVBX_LB, TVBX_WC)
+#undef TVBX_LB
+      } else {
+        const char *rk = reinterpret_cast<const char *>(ring) +
+                         (size_t)kk * a.n_node * rowb;
+#define TVBX_LG(o) tvbx_rowload(rk, (o), rowb, lane_off)
+        TVBX_GATHER_LD(nc, TVBX_XC, TVBX_LG, TVBX_WC)
+#undef TVBX_LG
+      }
+#undef TVBX_XC
+#undef TVBX_WC
+    }
+#endif
 #undef TVBX_CD0
 #undef TVBX_W0
 #undef TVBX_CD1
@@ -1063,25 +1097,55 @@ tvbx_step_blocked(const tvbx_step_args a) {
 }
 
 // Batched sweep of subjects with different connectomes (sizes, delays) in
-// one launch: workgroup w takes work item w (subject, instance tile, first
-// node; the host orders the items longest first, and the hardware hands
-// workgroups to free CUs in that order) and runs the blocked launch for
-// its nodes with that subject's arguments.  Every subject's delays allow
-// the launch's n_steps (checked by the host).
-extern "C" __global__ void __launch_bounds__(512, TVBX_MIN_WAVES)
-tvbx_step_batch(const tvbx_batch_args B) {
-  const int w = blockIdx.x;
+// one launch: a work item (subject, instance tile, first node; the host
+// orders the items longest first) runs the blocked launch for its nodes with
+// that subject's arguments.  Every subject's delays allow the launch's
+// n_steps (checked by the host).  Items are independent within a launch.
+__device__ __forceinline__ void tvbx_batch_item(const tvbx_batch_args &B,
+                                                int w, long long step0) {
   const int subj = __builtin_amdgcn_readfirstlane(B.items[4 * w + 0]);
   const int tile = __builtin_amdgcn_readfirstlane(B.items[4 * w + 1]);
   const int node0 = __builtin_amdgcn_readfirstlane(B.items[4 * w + 2]);
   tvbx_step_args a = B.subj[subj];
-  a.step0 = B.step0;
+  a.step0 = step0;
   a.n_steps = B.n_steps;
-  a.slot0 = (int)(B.step0 % a.horizon);
+  a.slot0 = (int)(step0 % a.horizon);
   a.clock = nullptr;
   const int i = node0 + (int)(threadIdx.x >> 6);
   if (i >= a.n_node || i >= node0 + a.node_block) return;
   tvbx_blocked_run(a, tile, i);
+}
+
+// Without a queue workgroup w takes item w, and the hardware hands
+// workgroups to free CUs in item order.  With a queue the grid holds as many
+// workgroups as can be resident, and each claims the next item with one
+// vector atomic add, runs it, and claims again until the items are gone: a
+// work queue that never waits (the items of a launch do not depend on each
+// other), so every workgroup exits once the last item is claimed.
+extern "C" __global__ void __launch_bounds__(512, TVBX_MIN_WAVES)
+tvbx_step_batch(const tvbx_batch_args B) {
+  const long long step0 = B.step0 + (B.clock != nullptr ? B.clock[0] : 0);
+  if (B.queue == nullptr) {
+    tvbx_batch_item(B, blockIdx.x, step0);
+    return;
+  }
+  __shared__ int claimed;
+  for (;;) {
+    if (threadIdx.x == 0) claimed = (int)atomicAdd(&B.queue[0], 1u);
+    __syncthreads();
+    const int w = claimed;
+    __syncthreads();
+    if (w >= B.n_items) break;
+    tvbx_batch_item(B, w, step0);
+  }
+  // the last workgroup to find the queue empty clears it for the next
+  // launch on the stream (nothing claims from it any more)
+  if (threadIdx.x == 0) {
+    if (atomicAdd(&B.queue[1], 1u) == gridDim.x - 1) {
+      atomicExch(&B.queue[0], 0u);
+      atomicExch(&B.queue[1], 0u);
+    }
+  }
 }
 #endif
 

@@ -125,6 +125,13 @@ typedef struct tvbx_batch_args {
   long long step0;
   int n_steps;
   int n_items;
+  /* graph replays: the launch's first step is step0 + clock[0] */
+  const long long *clock;
+  /* persistent workgroups: non-null when each workgroup claims items one
+   * after another from queue[0] (a vector atomic counter); queue[1] counts
+   * the workgroups that found the queue empty, and the last of them sets
+   * both back to 0 for the next launch.  Null: workgroup w runs item w. */
+  unsigned *queue;
 } tvbx_batch_args;
 
 /* Argument block of the adjoint (reverse) kernel.  `f` carries the forward

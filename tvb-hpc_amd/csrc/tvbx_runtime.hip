@@ -2276,11 +2276,22 @@ struct tvbx_batch {
   int noise_mode = 0;
   unsigned long long seed = 0;
   DevBuf row, coldelay, weight, state, ring, p_inst, p_node, p_ni, p_glob, bad,
-      args, items;
+      args, items, queue, clock;
   std::vector<tvbx_step_args> h_args;
   int n_items = 0;
   bool args_dirty = true;
+  // execution: persistent work-queue workgroups, captured graph windows
+  bool persistent = false;
+  bool use_graph = false;
+  long long graph_window = 0;
+  bool clock_dirty = true;
+  std::vector<std::pair<long long, hipGraphExec_t>> graphs;
 };
+
+static void batch_drop_graphs(tvbx_batch *b) {
+  for (auto &g : b->graphs) hipGraphExecDestroy(g.second);
+  b->graphs.clear();
+}
 
 static int batch_fill(tvbx_batch *b, DevBuf &buf, float v) {
   size_t n = buf.bytes / 4;
@@ -2439,7 +2450,9 @@ extern "C" int tvbx_batch_destroy(tvbx_batch *b) {
   hipStreamSynchronize(b->ctx->stream);
   DevBuf *all[] = {&b->row,    &b->coldelay, &b->weight, &b->state,
                    &b->ring,   &b->p_inst,   &b->p_node, &b->p_ni,
-                   &b->p_glob, &b->bad,      &b->args,   &b->items};
+                   &b->p_glob, &b->bad,      &b->args,   &b->items,
+                   &b->queue,  &b->clock};
+  batch_drop_graphs(b);
   for (DevBuf *x : all) dfree(*x);
   delete b;
   return TVBX_OK;
@@ -2520,6 +2533,7 @@ extern "C" int tvbx_batch_set_connectome(tvbx_batch *b, int k,
   }
   b->have_conn[k] = 1;
   b->n_items = 0;  // rebuilt at the next run
+  batch_drop_graphs(b);
   return TVBX_OK;
 }
 
@@ -2617,6 +2631,20 @@ extern "C" int tvbx_batch_set_step(tvbx_batch *b, long long step) {
   CHECK(b && step >= 0, "bad argument");
   b->step = step;
   b->args_dirty = true;
+  b->clock_dirty = true;
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_batch_set_exec(tvbx_batch *b, int persistent,
+                                   int use_graph, int graph_window) {
+  CHECK(b && (persistent == 0 || persistent == 1) &&
+            (use_graph == 0 || use_graph == 1) && graph_window >= 1,
+        "bad argument");
+  if ((persistent != 0) != b->persistent || graph_window != b->graph_window)
+    batch_drop_graphs(b);
+  b->persistent = persistent != 0;
+  b->use_graph = use_graph != 0;
+  b->graph_window = graph_window;
   return TVBX_OK;
 }
 
@@ -2653,6 +2681,67 @@ extern "C" int tvbx_batch_init_history(tvbx_batch *b, int mode) {
   return TVBX_OK;
 }
 
+// grid of one batch launch: an item per workgroup, or with the work queue
+// as many workgroups as the device holds at once (never more than items)
+static int batch_grid(tvbx_batch *b, unsigned *grid) {
+  if (!b->persistent) {
+    *grid = (unsigned)b->n_items;
+    return TVBX_OK;
+  }
+  int per_cu = 0;
+  HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, b->mod->batch, 64 * b->d.waves, 0));
+  long long g = (long long)std::max(per_cu, 1) *
+                std::max(b->ctx->prop.multiProcessorCount, 1);
+  *grid = (unsigned)std::min<long long>(g, b->n_items);
+  return TVBX_OK;
+}
+
+static void batch_launch_args(tvbx_batch *b, tvbx_batch_args &B,
+                              long long step0, int n_steps,
+                              const long long *clock) {
+  B.subj = (const tvbx_step_args *)b->args.p;
+  B.items = (const int *)b->items.p;
+  B.step0 = step0;
+  B.n_steps = n_steps;
+  B.n_items = b->n_items;
+  B.clock = clock;
+  B.queue = b->persistent ? (unsigned *)b->queue.p : nullptr;
+}
+
+// one graph window of W steps (a multiple of steps_per_launch): its launches
+// take their first step from the device clock, which the last node advances
+static int batch_capture(tvbx_batch *b, long long W, hipGraphExec_t *out) {
+  const int L = b->d.steps_per_launch;
+  unsigned grid = 0;
+  int rc = batch_grid(b, &grid);
+  if (rc != TVBX_OK) return rc;
+  hipStream_t st = b->ctx->stream;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  for (long long k = 0; k < W && e == hipSuccess; k += L) {
+    tvbx_batch_args B;
+    batch_launch_args(b, B, k, L, (const long long *)b->clock.p);
+    void *params[] = {&B};
+    e = hipModuleLaunchKernel(b->mod->batch, grid, 1, 1, 64u * b->d.waves,
+                              1, 1, 0, st, params, nullptr);
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st,
+                       (long long *)b->clock.p, W, 1);
+    e = hipGetLastError();
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(st, &g);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  if (g) hipGraphDestroy(g);
+  if (e != hipSuccess)
+    return fail(TVBX_E_HIP, "batch graph capture of %lld steps: %s", W,
+                hipGetErrorString(e));
+  return TVBX_OK;
+}
+
 extern "C" int tvbx_batch_run(tvbx_batch *b, long long n_steps) {
   CHECK(b && n_steps >= 0, "bad argument");
   for (int k = 0; k < b->d.n_subject; ++k)
@@ -2661,21 +2750,61 @@ extern "C" int tvbx_batch_run(tvbx_batch *b, long long n_steps) {
   int rc = TVBX_OK;
   if (b->n_items == 0) rc = batch_items(b);
   if (rc == TVBX_OK) rc = batch_upload_args(b);
+  if (rc == TVBX_OK && b->persistent && !b->queue.p) {
+    rc = dalloc(b->queue, 2 * sizeof(unsigned));
+    if (rc == TVBX_OK) {
+      // zeroed once; every launch leaves it zeroed again
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, b->ctx->stream,
+                         (unsigned *)b->queue.p, (size_t)2, 0u);
+      HIPCHK(hipGetLastError());
+    }
+  }
   if (rc != TVBX_OK) return rc;
   const int L = b->d.steps_per_launch;
+  long long W = 0;
+  if (b->use_graph) {
+    W = std::max<long long>(L, b->graph_window - b->graph_window % L);
+    if (!b->clock.p) {
+      rc = dalloc(b->clock, 2 * sizeof(long long));
+      if (rc != TVBX_OK) return rc;
+      b->clock_dirty = true;
+    }
+  }
+  unsigned grid = 0;
+  rc = batch_grid(b, &grid);
+  if (rc != TVBX_OK) return rc;
   for (long long done = 0; done < n_steps;) {
+    if (W > 0 && n_steps - done >= W) {
+      if (b->clock_dirty) {
+        long long h[2] = {b->step, 0};
+        HIPCHK(hipMemcpyAsync(b->clock.p, h, sizeof h, hipMemcpyHostToDevice,
+                              b->ctx->stream));
+        HIPCHK(hipStreamSynchronize(b->ctx->stream));
+        b->clock_dirty = false;
+      }
+      hipGraphExec_t ex = nullptr;
+      for (auto &g : b->graphs)
+        if (g.first == W) ex = g.second;
+      if (!ex) {
+        rc = batch_capture(b, W, &ex);
+        if (rc != TVBX_OK) return rc;
+        b->graphs.push_back({W, ex});
+      }
+      HIPCHK(hipGraphLaunch(ex, b->ctx->stream));
+      b->step += W;
+      done += W;
+      continue;
+    }
     tvbx_batch_args B;
-    B.subj = (const tvbx_step_args *)b->args.p;
-    B.items = (const int *)b->items.p;
-    B.step0 = b->step;
-    B.n_steps = (int)std::min<long long>(L, n_steps - done);
-    B.n_items = b->n_items;
+    batch_launch_args(b, B, b->step, (int)std::min<long long>(L, n_steps - done),
+                      nullptr);
     void *params[] = {&B};
-    HIPCHK(hipModuleLaunchKernel(b->mod->batch, b->n_items, 1, 1,
+    HIPCHK(hipModuleLaunchKernel(b->mod->batch, grid, 1, 1,
                                  64u * b->d.waves, 1, 1, 0, b->ctx->stream,
                                  params, nullptr));
     b->step += B.n_steps;
     done += B.n_steps;
+    b->clock_dirty = true;  // eager launches advance the host step only
   }
   // the per-subject step0 / slot0 in the table are stale now; the batch
   // kernel takes them from B, but init_history uses them

@@ -254,29 +254,43 @@ def best_batch_launch(cfg, ctx=None):
     return default_batch_launch(cfg)
 
 
-def tune_batch(cfg, steps=200, warmup=20, ctx=None, log=None):
-    """Time the batch launch shapes; record and return the best."""
+def _time_batch(cfg, cand, x0, steps, warmup, ctx):
+    b = cfg.batch(context=ctx, **cand)
+    for k, x in enumerate(x0):
+        b.set_state(k, x)
+    b.init_history('zeros')
+    b.set_noise(seed=1)
+    b.set_exec(graph=True, window=100)
+    b.run(warmup)
+    b.sync()
+    t = time.perf_counter()
+    b.run(steps)
+    b.sync()
+    ms = (time.perf_counter() - t) / steps * 1e3
+    del b
+    return ms
+
+
+def tune_batch(cfg, steps=200, warmup=20, ctx=None, log=None, top=6):
+    """Time the batch launch shapes (graph windows of 100 steps), then the
+    `top` best again with persistent work-queue workgroups; record and
+    return the best."""
     from .hip import Context
     ctx = ctx or Context.default()
     x0 = cfg.initial_states()
     results = []
     cands = batch_candidates(cfg)
     # the first shape twice: its first, untimed pass raises the clocks
-    for it, cand in enumerate([cands[0]] + cands):
-        b = cfg.batch(context=ctx, **cand)
-        for k, x in enumerate(x0):
-            b.set_state(k, x)
-        b.init_history('zeros')
-        b.set_noise(seed=1)
-        b.run(warmup)
-        b.sync()
-        t = time.perf_counter()
-        b.run(steps)
-        b.sync()
-        ms = (time.perf_counter() - t) / steps * 1e3
-        del b
-        if it == 0:
-            continue
+    _time_batch(cfg, cands[0], x0, steps, warmup, ctx)
+    for cand in cands:
+        ms = _time_batch(cfg, cand, x0, steps, warmup, ctx)
+        results.append((ms, cand))
+        if log:
+            log('%s %.4f ms/step' % (cand, ms))
+    results.sort(key=lambda r: r[0])
+    for _, base in list(results[:top]):
+        cand = dict(base, queue=True)
+        ms = _time_batch(cfg, cand, x0, steps, warmup, ctx)
         results.append((ms, cand))
         if log:
             log('%s %.4f ms/step' % (cand, ms))

@@ -9,12 +9,13 @@ sum by the number of nodes, not by the in-degree (network.py:129-131).
 """
 
 import enum
+import logging
 
 import numpy as np
 
 from . import expr as E
 from .model import BaseModel
-from .utils import exprs, getLogger
+from .utils import exprs
 
 
 class PostSumStat(enum.Enum):
@@ -42,7 +43,7 @@ class BaseCoupling:
         self.pre_sum_sym = exprs(self.pre_sum)
         self.post_sum_sym = exprs(self.post_sum)
         lname = '%s<%s>' % (type(self).__name__, type(model).__name__)
-        self.logger = getLogger(lname)
+        self.logger = logging.getLogger(lname)
         self._check_io()
 
     def _check_io(self):

@@ -19,6 +19,7 @@ raise.
 """
 
 import ctypes
+import logging
 import os
 import sys
 import threading
@@ -28,9 +29,8 @@ import numpy as np
 from . import emit
 from . import expr as E
 from .network import CSR, horizon_for
-from .utils import getLogger
 
-LOG = getLogger(__name__)
+LOG = logging.getLogger(__name__)
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, '_lib', 'libtvbx.so')
 KDIR = os.path.join(HERE, '_lib', 'kernels')
@@ -128,6 +128,7 @@ _SIGS = {
     'tvbx_batch_get_step': [_P, _P],
     'tvbx_batch_set_noise': [_P, _I, ctypes.c_ulonglong],
     'tvbx_batch_init_history': [_P, _I],
+    'tvbx_batch_set_exec': [_P, _I, _I, _I],
     'tvbx_batch_run': [_P, _LL],
     'tvbx_batch_sync': [_P],
     'tvbx_batch_get_flags': [_P, _P],
@@ -788,14 +789,15 @@ class Batch:
     params: {name: value}; instance parameters as (P_total,) arrays, node
         parameters as a list of (N_k,) arrays, scalars for globals.
     launch: ipl, waves (= nodes per workgroup), steps_per_launch, min_waves,
-        unroll, pipe, prio (see tvb_hpc.autotune).
+        unroll, pipe, prio, queue (persistent work-queue workgroups; see
+        tvb_hpc.autotune).
     Philox noise only (``set_noise``); no monitor.
     """
 
     def __init__(self, model, cfun, scheme, subjects, params=None,
                  kinds=None, ring_dtype='f32', ring_scale=1.0, waves=1,
                  steps_per_launch=1, ipl=1, min_waves=1, unroll=8, pipe=False,
-                 prio=True, dt=None,
+                 prio=True, queue=False, dt=None,
                  context=None):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.subjects = [(csr, int(p)) for csr, p in subjects]
@@ -859,6 +861,8 @@ class Batch:
                 self.h, k, _ptr(csr.row), _ptr(csr.col), _ptr(csr.weight),
                 _ptr(np.ascontiguousarray(csr.delays[0], np.int32))))
         self.set_params(resolved)
+        self.queue = bool(queue)
+        self.set_exec(graph=False, queue=self.queue)
 
     def __del__(self):
         h = getattr(self, 'h', None)
@@ -940,6 +944,14 @@ class Batch:
             check(lib().tvbx_batch_set_noise(self.h, 0, 0))
         else:
             check(lib().tvbx_batch_set_noise(self.h, 1, int(seed)))
+
+    def set_exec(self, graph=True, window=100, queue=None):
+        """graph: replay captured windows of `window` steps; queue:
+        persistent work-queue workgroups (default: as constructed)."""
+        if queue is not None:
+            self.queue = bool(queue)
+        check(lib().tvbx_batch_set_exec(self.h, int(self.queue),
+                                        int(bool(graph)), int(window)))
 
     def run(self, n_steps):
         check(lib().tvbx_batch_run(self.h, int(n_steps)))

@@ -19,14 +19,15 @@ This module also holds the seeded synthetic-connectome generator used by the
 sweep configurations and the integer delay computation.
 """
 
+import logging
+
 import numpy as np
 from scipy import sparse
 
 from .model import BaseModel
 from .coupling import BaseCoupling
-from .utils import getLogger
 
-LOG = getLogger(__name__)
+LOG = logging.getLogger(__name__)
 
 
 class Connectivity:

@@ -10,12 +10,14 @@ nnz, delays, row, col, weights, <coupling params>, i_step_0); the ring slot of
 step n is (n) % ntime as in transforms.py:45-46.
 """
 
+import logging
+
 from . import utils
 from .coupling import BaseCoupling
 from .model import BaseModel
 from .scheme import TimeStepScheme
 
-LOG = utils.getLogger('tvb_hpc')
+LOG = logging.getLogger('tvb_hpc')
 
 
 def network_time_step(model: BaseModel, coupling: BaseCoupling,

@@ -1,48 +1,21 @@
 """
-Helpers mirroring tvb_hpc/utils.py (reference): symbolic ``vars`` / ``exprs``,
-variable substitution, a wall-clock ``timer`` and logging.
+Helpers with the names of tvb_hpc/utils.py (reference): symbolic ``vars`` /
+``exprs``, variable substitution, a wall-clock ``timer``.
 
 The reference's ``default_target`` (utils.py:32) is an OpenCL target; here the
 default target is the HIP executor (``tvb_hpc.hip.HipTarget``).
 """
 
 import logging
-import os
 import time
 
 import numpy as np
 
 from . import expr as _e
 
-here = os.path.dirname(os.path.abspath(__file__))
-
-
 def default_target():
     from .hip import HipTarget
     return HipTarget()
-
-
-def can_bcast(n, m):
-    return (n == 1) or (m == 1) or (n == m)
-
-
-def getLogger(name):
-    return logging.getLogger(name)
-
-
-class NoSuchExecutable(RuntimeError):
-    "Exception raised when ``which`` does not find its argument on $PATH."
-
-
-def which(exe):
-    "Find exe name on $PATH (utils.py:43-56)."
-    if os.path.exists(exe):
-        return exe
-    for path in os.environ['PATH'].split(os.path.pathsep):
-        maybe_path = os.path.join(path, exe)
-        if os.path.exists(maybe_path):
-            return maybe_path
-    raise NoSuchExecutable(exe)
 
 
 def parse(text):
@@ -76,19 +49,23 @@ def exprs(sexprs):
 
 
 class timer:
-    "Wall-clock context manager logging elapsed time (utils.py:93-107)."
-    logger = getLogger('tvb_hpc.utils.timer')
+    """Measures the wall time of a ``with`` block with the monotonic
+    performance counter; the seconds are kept in ``elapsed`` and reported
+    at DEBUG level on the 'tvb_hpc' logger."""
 
     def __init__(self, msg=''):
         self.msg = msg
+        self.elapsed = None
 
-    def __enter__(self, *args):
-        self.tic = time.time()
+    def __enter__(self):
+        self._t0 = time.perf_counter()
         return self
 
-    def __exit__(self, *args):
-        self.elapsed = time.time() - self.tic
-        self.logger.info('%s elapsed %.3fs', self.msg, self.elapsed)
+    def __exit__(self, *exc):
+        self.elapsed = time.perf_counter() - self._t0
+        logging.getLogger('tvb_hpc').debug('%s: %.6f s', self.msg or 'timer',
+                                           self.elapsed)
+        return False
 
 
 class VarSubst:
