@@ -17,8 +17,10 @@ value        instance*steps/s of the batched mixed-precision sweep at N=998
              (Jansen-Rit, P=1024, delays quantised to 32 bins, bfloat16
              history of the 1/cmax-scaled sigmoid, FP32 state and sums, FP64
              FC): P * K / wall time
-variants     the same sweep with a float16 history and with an 8-bit
-             fixed-point history
+variants     the same sweep with a float16 history, with an 8-bit
+             fixed-point history, and with the delay-binned sum over bins
+             of W_d X(t - d) on the matrix cores (bfloat16 / float16 MFMA,
+             float32 accumulation) instead of the CSR row gathers
 plain        the same sweep through the plain FP32 kernel (exact delays,
              float32 history); vs_plain = value / plain value (wall clock),
              vs_plain_kernel the same ratio of step-kernel times
@@ -532,6 +534,13 @@ def bench_batch(args):
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
         'data': 'synthetic',
         'config': {'workload': '%s: %s' % (cfg.name, cfg.description),
+                   'reading': '2048 instances = %d subjects x %d instances: '
+                              'each subject its own connectome (N drawn '
+                              'from the set, own seed, own conduction '
+                              'speed U(1, 10) m/s), its instances sweeping '
+                              'G, J_NMDA and sigma; not one connectome per '
+                              'instance' % (len(cfg.subjects),
+                                            cfg.subjects[0][1]),
                    'subjects': len(cfg.subjects), 'n_inst': cfg.n_inst,
                    'node_counts': Ns, 'nnz_total': sum(
                        c.nnz for c, _ in cfg.subjects),
@@ -617,7 +626,9 @@ def main():
                     help='timed steps (default: the workload\'s run length)')
     ap.add_argument('--warmup', type=int, default=200)
     ap.add_argument('--workload', default='jansenrit998_bf16')
-    ap.add_argument('--variants', default='jansenrit998_f16,jansenrit998_u8',
+    ap.add_argument('--variants',
+                    default='jansenrit998_f16,jansenrit998_u8,'
+                            'jansenrit998_bf16_mfma,jansenrit998_f16_mfma',
                     help='other mixed-precision workloads timed alongside '
                          "(comma separated, or 'none')")
     ap.add_argument('--plain', default='jansenrit998',

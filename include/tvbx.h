@@ -180,6 +180,17 @@ int tvbx_sim_set_noise(tvbx_sim *sim, int mode, unsigned long long seed);
  * n_inner_steps from the host, cuda_run.py:124-142.) */
 int tvbx_sim_set_exec(tvbx_sim *sim, int use_graph, int graph_window,
                       int profile);
+/* Coupling method.  Reference: the coupling sum of network.py:115-136 over
+ * CSR rows (mode 0, the default).  Mode 1 computes the delay-binned sum
+ * sum_d W_d X(t - d) on the matrix cores (bfloat16 / float16 MFMA, float32
+ * accumulation): targets in blocks of 16, each block's distinct (delay,
+ * column) pairs in chunks of 32, weights split exactly into high and low
+ * 16-bit parts.  Needs a module whose ring is bfloat16 or float16 holding
+ * pre(y), the CUDA ordering, a blocked launch shape (node_block == waves
+ * <= 8 < n_node) of at most 16 steps, one delay and weight row, a ring
+ * under 4 GiB; tvbx_sim_logp, tvbx_sim_checkpoint_run and tvbx_sim_adjoint
+ * refuse mode 1.  The tables are rebuilt by tvbx_sim_set_connectome. */
+int tvbx_sim_set_coupling(tvbx_sim *sim, int mode);
 /* advance n_steps.  noise: optional injected normals (host), either
  * [n_steps][S][N][P] (tstride = S*N*P) or [S][N][P] reused each step
  * (tstride = 0). */

@@ -92,12 +92,15 @@ def test_config1_g2do84_full_sweep(ctx):
         err = state_err('Generic2dOscillator', x[..., sl], ref.get_state())
         ref_s = out['samples']
         assert smp.shape[0] == ref_s.shape[0] == STEPS // cfg.tavg_period
-        # BOLD starts at 0 (the rest state): error relative to its range
-        err_b = float(np.abs(smp[..., sl] - ref_s).max() /
-                      np.abs(ref_s).max())
-        print('g2do84 %s instances %d-%d: state %.2e, BOLD %.2e'
-              % (la, sl.start, sl.stop - 1, err, err_b))
-        assert err < TOL and err_b < TOL
+        # BOLD = V0 (k1 (1 - q) + k2 (1 - q/v) + k3 (1 - v)) with q, v ~ 1
+        # and V0 k1 ~ 11: one float32 ulp of q (from powf on either side)
+        # moves the signal by 1.3e-6 whatever its size (here below 1e-3,
+        # where float32 and float64 BOLD differ by ~1e-5): an absolute bound
+        err_b = float(np.abs(smp[..., sl] - ref_s).max())
+        print('g2do84 %s instances %d-%d: state %.2e, |dBOLD| %.2e of '
+              'max |BOLD| %.2e' % (la, sl.start, sl.stop - 1, err, err_b,
+                                   np.abs(ref_s).max()))
+        assert err < TOL and err_b < 4e-6
     fc = sim.fc(slices[-1].start, 4)
     fc_ref = O.fc_f64(smp[:, :, slices[-1].start:slices[-1].start + 4]
                       .astype(np.float64))
@@ -159,7 +162,9 @@ JR_SLICE = slice(496, 512)
 @pytest.mark.parametrize('name,tol', [('jansenrit998', TOL),
                                       ('jansenrit998_bf16', 1e-4),
                                       ('jansenrit998_f16', 1e-4),
-                                      ('jansenrit998_u8', 1e-4)])
+                                      ('jansenrit998_u8', 1e-4),
+                                      ('jansenrit998_bf16_mfma', 1e-4),
+                                      ('jansenrit998_f16_mfma', 1e-4)])
 def test_config3_jansenrit998_trajectories(ctx, name, tol):
     """N = 998, P = 1024, tuned launch, 1000 steps (deterministic Euler).
     Plain float32: the ring is over 4 GiB, rows hold 64-127 edges.  The
@@ -199,7 +204,9 @@ def jr_fc64():
 
 
 @pytest.mark.parametrize('name', ['jansenrit998', 'jansenrit998_bf16',
-                                  'jansenrit998_f16', 'jansenrit998_u8'])
+                                  'jansenrit998_f16', 'jansenrit998_u8',
+                                  'jansenrit998_bf16_mfma',
+                                  'jansenrit998_f16_mfma'])
 def test_config3_fc_after_full_run(ctx, jr_fc64, name):
     """FC (FP64 co-moments on the device) after the 10,000-step run with
     graph windows, against the float64 oracle with exact delays."""

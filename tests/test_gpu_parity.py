@@ -648,3 +648,93 @@ def test_long_rows_match_oracle(ipl, big_ring, launch):
     print('long rows (%d-%d edges) %s big_ring=%s: %.2e'
           % (deg.min(), deg.max(), la, big_ring, err))
     assert err < TOL
+
+
+@pytest.mark.parametrize('ring_dtype', ['bf16', 'f16'])
+@pytest.mark.parametrize('launch', [
+    dict(ipl=1, waves=4, node_block=4, steps_per_launch=8),
+    dict(ipl=4, waves=2, node_block=2, steps_per_launch=1)])
+def test_mfma_coupling_parity(ring_dtype, launch):
+    """Delay-binned sums on the matrix cores (16-target blocks, 32-pair
+    chunks, weights as high + low 16-bit parts) against the float32 oracle
+    emulating the ring's rounding, with injected noise: within 1e-4, as the
+    other reduced-precision rings (the products are exact; the sums are
+    accumulated in another order and the split weights carry 16 (bf16) or
+    22 (f16) significant bits)."""
+    sim, ref, noise = _pair('JansenRit', 'Sigmoidal', 0.05, {'a': 0.3},
+                            P=200, steps=600, ring_dtype=ring_dtype,
+                            ring_scale=200.0, launch=launch)
+    assert int(sim.csr.delays.min()) >= launch['steps_per_launch'] - 1
+    sim.set_coupling('mfma')
+    sim.run(600, noise=noise)
+    ref.run(600, noise=noise)
+    err = state_err('JansenRit', sim.get_state(), ref.get_state())
+    print('JR %s ring, MFMA coupling %s vs emulating oracle: %.2e'
+          % (ring_dtype, launch, err))
+    assert err < 1e-4
+    assert not sim.flags().any()
+
+
+def test_mfma_coupling_graphs_equal_eager():
+    """Captured graph windows replay the two-kernel (sums, advance) launches
+    bit for bit."""
+    out = []
+    for graph in (False, True):
+        sim, ref, noise = _pair('JansenRit', 'Sigmoidal', 0.05, {'a': 0.3},
+                                P=128, steps=10, em=False, ring_dtype='bf16',
+                                ring_scale=200.0,
+                                launch=dict(waves=4, node_block=4,
+                                            steps_per_launch=8))
+        sim.set_coupling('mfma')
+        sim.set_exec(graph=graph, window=16)
+        sim.run(8 * 13 + 5)
+        out.append(sim.get_state())
+    np.testing.assert_array_equal(out[0], out[1])
+
+
+def test_mfma_coupling_refused_where_it_does_not_apply():
+    sim, _, _ = _pair('JansenRit', 'Sigmoidal', 0.05, {'a': 0.3}, P=64,
+                      steps=1, launch=dict(waves=4, node_block=4,
+                                           steps_per_launch=4))
+    with pytest.raises(TvbxError):
+        sim.set_coupling('mfma')     # float32 ring: no matrix-core kernel
+
+
+@pytest.mark.parametrize('ring_dtype,ipl', [('f32', 1), ('bf16', 4),
+                                            ('bf16', 2), ('u8', 4)])
+def test_edge_major_blocked_equals_step_major(ring_dtype, ipl):
+    """Blocked launches that form all their steps' coupling sums first
+    (edges outer, steps inner) give the state of the step-by-step blocked
+    launches bit for bit, including a launch shorter than the block."""
+    out = []
+    for emaj in (0, 8):
+        sim, _, _ = _pair('JansenRit', 'Sigmoidal', 0.05, {'a': 0.3}, P=256,
+                          steps=1, em=False, ring_dtype=ring_dtype,
+                          ring_scale=1.0 if ring_dtype == 'f32' else 200.0,
+                          launch=dict(ipl=ipl, waves=2, node_block=2,
+                                      steps_per_launch=8, emaj=emaj))
+        assert int(sim.csr.delays.min()) >= 7
+        sim.run(8 * 10 + 3)
+        out.append(sim.get_state())
+        assert not sim.flags().any()
+    np.testing.assert_array_equal(out[0], out[1])
+
+
+def test_edge_major_long_rows_equal_step_major():
+    """The same for rows of 188-232 edges (chunks past the register rows)."""
+    N, P, dt = 300, 64, 0.1
+    csr = small_csr(N=N, density=0.7, seed=23, dt=dt, lmax=40.0)
+    m = MODELS['ReducedWongWang']()
+    c = CFUNS['Linear'](m)
+    out = []
+    for emaj in (0, 4):
+        sim = Simulation(m, c, SC.EulerStep(dt), csr, P,
+                         params={'c:a': np.full(P, 0.02, np.float32)},
+                         waves=1, node_block=1, steps_per_launch=4,
+                         emaj=emaj)
+        x0 = m.initial_state(N, P, np.random.default_rng(3))
+        sim.set_state(x0)
+        sim.init_history('zeros')
+        sim.run(4 * 25 + 2)
+        out.append(sim.get_state())
+    np.testing.assert_array_equal(out[0], out[1])

@@ -321,7 +321,8 @@ tvbx_adjoint(const tvbx_adj_args A) {
 //   d log p / d theta_m = sum_s lam_s d f_s / d theta_m        (model)
 //   d log p / d theta_c = [sum_s lam_s d f_s / d c] d post / d theta_c
 // with the emitted vector-Jacobian products; c(n) = post(sum(n)) from the
-// sums the forced steps recorded.  FP64 accumulation over the window.
+// sums the forced steps recorded.  Compensated float32 sums over the window,
+// added into the compensated sums of the earlier windows.
 // grid (pitch / (64 ipl), N), block 64.
 extern "C" __global__ void __launch_bounds__(64)
 tvbx_logp_grad(const tvbx_logp_args L) {
@@ -333,13 +334,14 @@ tvbx_logp_grad(const tvbx_logp_args L) {
   tvbx_lane_params lp[TVBX_IPL];
 #pragma unroll
   for (int q = 0; q < TVBX_IPL; ++q) tvbx_load_lane(lp[q], a, pb + q);
-  double gm_acc[TVBX_IPL][TVBX_NGM], gc_acc[TVBX_IPL][TVBX_NGC];
+  float gm_acc[TVBX_IPL][TVBX_NGM], gc_acc[TVBX_IPL][TVBX_NGC];
+  float gm_err[TVBX_IPL][TVBX_NGM], gc_err[TVBX_IPL][TVBX_NGC];
 #pragma unroll
   for (int q = 0; q < TVBX_IPL; ++q) {
 #pragma unroll
-    for (int g = 0; g < TVBX_NGM; ++g) gm_acc[q][g] = 0.0;
+    for (int g = 0; g < TVBX_NGM; ++g) gm_acc[q][g] = gm_err[q][g] = 0.0f;
 #pragma unroll
-    for (int g = 0; g < TVBX_NGC; ++g) gc_acc[q][g] = 0.0;
+    for (int g = 0; g < TVBX_NGC; ++g) gc_acc[q][g] = gc_err[q][g] = 0.0f;
   }
   for (int k = 0; k < a.n_steps; ++k) {
     tvbx_progress_prio(k, a.n_steps);
@@ -364,23 +366,38 @@ tvbx_logp_grad(const tvbx_logp_args L) {
       tvbx_dfun_vjp(x, c, lam, jt, muc, gm, lp[q], a, i, p);
       tvbx_dpost_dparam(sum, gc, lp[q], a);
 #pragma unroll
-      for (int r = 0; r < TVBX_NGM; ++r) gm_acc[q][r] += (double)gm[r];
+      for (int r = 0; r < TVBX_NGM; ++r)
+        tvbx_csum_add(gm_acc[q][r], gm_err[q][r], gm[r]);
 #pragma unroll
       for (int r = 0; r < TVBX_NGC; ++r)
-        gc_acc[q][r] += (double)muc[0] * (double)gc[r];
+        tvbx_csum_add(gc_acc[q][r], gc_err[q][r], muc[0] * gc[r]);
     }
   }
+  // into the planes of the earlier windows' sums: plane 0 += the window's
+  // sum (compensated), plane 1 += both rounding errors
 #pragma unroll
   for (int q = 0; q < TVBX_IPL; ++q) {
 #if TVBX_HAS_NGM
 #pragma unroll
-    for (int r = 0; r < TVBX_NGM; ++r)
-      L.g_m[(size_t)r * NP + ip + q] += gm_acc[q][r];
+    for (int r = 0; r < TVBX_NGM; ++r) {
+      float *gs = L.g_m + (size_t)r * NP + ip + q;
+      float *ge = gs + (size_t)L.n_gm * NP;
+      float sv = *gs, ev = *ge;
+      tvbx_csum_add(sv, ev, gm_acc[q][r]);
+      *gs = sv;
+      *ge = ev + gm_err[q][r];
+    }
 #endif
 #if TVBX_HAS_NGC
 #pragma unroll
-    for (int r = 0; r < TVBX_NGC; ++r)
-      L.g_c[(size_t)r * NP + ip + q] += gc_acc[q][r];
+    for (int r = 0; r < TVBX_NGC; ++r) {
+      float *gs = L.g_c + (size_t)r * NP + ip + q;
+      float *ge = gs + (size_t)L.n_gc * NP;
+      float sv = *gs, ev = *ge;
+      tvbx_csum_add(sv, ev, gc_acc[q][r]);
+      *gs = sv;
+      *ge = ev + gc_err[q][r];
+    }
 #endif
   }
 }

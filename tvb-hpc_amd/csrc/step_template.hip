@@ -14,6 +14,8 @@
 //   TVBX_MIN_WAVES   waves per SIMD the blocked kernel's registers allow
 //   TVBX_PIPE        (defined) two gather groups in flight per wave
 //   TVBX_PRIO        1: wave priority falls with a multi-step launch's progress
+//   TVBX_EMAJ        L > 0: blocked launches of up to L steps form all their
+//                    coupling sums first, edges outer and steps inner
 //   struct tvbx_lane_params + tvbx_load_lane()   per-instance parameters
 //   tvbx_pre(), tvbx_post(), tvbx_dfun(), tvbx_obsrv(), tvbx_wrap(),
 //   tvbx_ring_value(), tvbx_bold_step()
@@ -252,6 +254,15 @@ __device__ __forceinline__ float tvbx_acc_unscale(const tvbx_step_args &a) {
 #else
   return a.ring_unscale;
 #endif
+}
+
+// s + c += x as a compensated float32 sum (Neumaier): c collects the
+// rounding error of each addition to s; the value is s + c.  Needs the
+// operations as written (no contraction).
+__device__ __forceinline__ void tvbx_csum_add(float &s, float &c, float x) {
+  const float t = s + x;
+  c += fabsf(s) >= fabsf(x) ? (s - t) + x : (x - t) + s;
+  s = t;
 }
 
 // normals for state variables [s0, s0 + 4) of (instance p, step n, node i)
@@ -695,6 +706,118 @@ __device__ __forceinline__ void tvbx_coupling(
   }
 }
 
+#if TVBX_EMAJ && !TVBX_ORDER_REF
+// Coupling sums of node i for all steps k < n of a blocked launch, edges
+// outer and steps inner.  Every read is of a slot written before the launch
+// (delays >= n - 1) and pre() does not read post_syn, so the sums of the
+// launch's steps do not depend on its updates and can all be formed first.
+// For each group of TVBX_UNROLL edges the slots of every step are gathered
+// back to back: the rows of one (source, delay) are then read by all their
+// targets' waves at about the same time, while they are in the L2 (rows are
+// sorted by delay, so every wave walks the delays in the same order).  The
+// additions of each step follow the row order and the groups of the per-step
+// gather, so the sums equal those of tvbx_coupling bit for bit.  Ring under
+// 4 GiB only (byte offsets from one buffer descriptor).
+__device__ __forceinline__ void tvbx_coupling_launch(
+    const tvbx_step_args &a, const unsigned *__restrict__ cd,
+    const float *__restrict__ wt, const tvbx_row &row,
+    const tvbx_rt *__restrict__ ring, int t, int pb, int n,
+    const tvbx_lane_params (&lp)[TVBX_IPL],
+    float (&c)[TVBX_EMAJ][TVBX_IPL][TVBX_C]) {
+  const int lane = threadIdx.x & 63;
+  const int deg = __builtin_amdgcn_readfirstlane(row.e1 - row.e0);
+  const unsigned rowb = (unsigned)a.pitch * (unsigned)sizeof(tvbx_rt);
+  const unsigned SB = (unsigned)(TVBX_R * a.n_node) * rowb;  // one slot
+  const unsigned HSB = (unsigned)a.horizon * SB;               // the ring
+  const __amdgpu_buffer_rsrc_t rs = tvbx_ring_rsrc(ring);
+  const unsigned lane_off = (unsigned)pb * (unsigned)sizeof(tvbx_rt);
+  float post[TVBX_IPL];
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q) post[q] = 0.0f;
+#pragma unroll
+  for (int kk = 0; kk < TVBX_C; ++kk) {
+    float acc[TVBX_EMAJ][TVBX_IPL];
+#pragma unroll
+    for (int k = 0; k < TVBX_EMAJ; ++k)
+#pragma unroll
+      for (int q = 0; q < TVBX_IPL; ++q) acc[k][q] = 0.0f;
+    const unsigned voff = lane_off + (unsigned)(kk * a.n_node) * rowb;
+    for (int cb = 0; cb < deg; cb += 64) {
+      const int nc = __builtin_amdgcn_readfirstlane(min(deg - cb, 64));
+      unsigned cdc;
+      float wc;
+      if (cb == 0) {
+        cdc = row.cd0;
+        wc = row.w0;
+      } else if (cb == 64) {
+        cdc = row.cd1;
+        wc = row.w1;
+      } else {
+        const int ec = row.e0 + cb + lane;
+        cdc = lane < nc ? cd[ec] : 0u;
+        wc = lane < nc ? wt[ec] : 0.0f;
+      }
+      // byte offsets of the edges' rows at the launch's first slot
+      const unsigned ro = tvbx_edge_row(cdc, t, a) * rowb;
+      for (int u0 = 0; u0 < nc; u0 += TVBX_UNROLL) {
+        unsigned so[TVBX_UNROLL];
+        float ws[TVBX_UNROLL];
+#pragma unroll
+        for (int u = 0; u < TVBX_UNROLL; ++u) {
+          const int e = min(u0 + u, nc - 1);
+          so[u] = tvbx_lane_u(ro, e);
+          ws[u] = u0 + u < nc ? tvbx_lane_f(wc, e) : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < TVBX_EMAJ; ++k) {
+          if (k < n) {
+            tvbx_raw v[TVBX_UNROLL];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < TVBX_UNROLL; ++u) {
+              unsigned o = so[u] + (unsigned)k * SB;
+              o = o >= HSB ? o - HSB : o;
+              v[u] = tvbx_bload(rs, voff, o);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < TVBX_UNROLL; ++u)
+              tvbx_accum(acc[k], ws[u], v[u], post, lp, a, kk);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < TVBX_EMAJ; ++k)
+#pragma unroll
+      for (int q = 0; q < TVBX_IPL; ++q)
+        c[k][q][kk] = tvbx_post(kk, acc[k][q] * tvbx_acc_unscale(a), lp[q], a);
+  }
+}
+#endif
+
+// Coupling of node i at step k of the launch from sums another kernel
+// computed (csum [n_steps][C][N][pitch]), finished as in tvbx_coupling.
+__device__ __forceinline__ void tvbx_coupling_from_sums(
+    const tvbx_step_args &a, int i, int pb, long long n, int k,
+    const tvbx_lane_params (&lp)[TVBX_IPL], float (&c)[TVBX_IPL][TVBX_C]) {
+  const size_t NP = (size_t)a.n_node * a.pitch;
+  const size_t ip = (size_t)i * a.pitch + pb;
+#pragma unroll
+  for (int kk = 0; kk < TVBX_C; ++kk) {
+    float acc[TVBX_IPL];
+    tvbx_ldf(a.csum + ((size_t)k * TVBX_C + kk) * NP + ip, acc);
+#pragma unroll
+    for (int q = 0; q < TVBX_IPL; ++q) {
+      acc[q] = acc[q] * tvbx_acc_unscale(a);
+      c[q][kk] = tvbx_post(kk, acc[q], lp[q], a);
+    }
+    if (a.rec_sum != nullptr)
+      tvbx_stf(a.rec_sum + ((size_t)(n - a.rec_step0) * TVBX_C + kk) * NP + ip,
+               acc);
+  }
+}
+
 // non-finite flags of the lane's instances (bit q: instance pb + q)
 __device__ __forceinline__ void tvbx_flag_nonfinite(
     const tvbx_step_args &a, int pb, const float (&x)[TVBX_IPL][TVBX_S],
@@ -735,17 +858,20 @@ __device__ __forceinline__ void tvbx_advance(
       // FP64; the state then takes the given value
       const float *__restrict__ xn =
           a.force + (size_t)(n + 1 - a.force_step0) * TVBX_S * NP + ip + q;
-      double lp_acc = 0.0;
+      float lp_step = 0.0f;
 #pragma unroll
       for (int s = 0; s < TVBX_S; ++s) {
         const float mu = x[q][s] + a.dt * f[s];
         const float xs1 = xn[(size_t)s * NP];
-        const double r = (double)(xs1 - mu);
-        const double sd2 = (double)a.dt * (double)g[s] * (double)g[s];
-        lp_acc -= r * r / (2.0 * sd2);
+        const float r = xs1 - mu;
+        const float sd2 = a.dt * g[s] * g[s];
+        lp_step -= r * r / (2.0f * sd2);
         x[q][s] = xs1;
       }
-      a.logp[ip + q] += lp_acc;
+      float ls = a.logp[ip + q], lc = a.logp[NP + ip + q];
+      tvbx_csum_add(ls, lc, lp_step);
+      a.logp[ip + q] = ls;
+      a.logp[NP + ip + q] = lc;
     } else {
       if (a.noise_mode == 1) {
 #pragma unroll
@@ -1049,6 +1175,13 @@ __device__ __forceinline__ void tvbx_blocked_run(const tvbx_step_args &a,
   // the node's row serves every step of the launch
   tvbx_row rw;
   tvbx_load_row(a, cd, wt, i, rw);
+#if TVBX_EMAJ
+  // all the launch's coupling sums first, edges outer and steps inner
+  const bool emaj = a.ring32 && a.csum == nullptr && a.force == nullptr &&
+                    a.rec_sum == nullptr && a.n_steps <= TVBX_EMAJ;
+  float ce[TVBX_EMAJ][TVBX_IPL][TVBX_C];
+  if (emaj) tvbx_coupling_launch(a, cd, wt, rw, ring, t, pb, a.n_steps, lp, ce);
+#endif
 
   for (int k = 0; k < a.n_steps; ++k) {
     tvbx_progress_prio(k, a.n_steps);
@@ -1058,7 +1191,25 @@ __device__ __forceinline__ void tvbx_blocked_run(const tvbx_step_args &a,
     float y_mon[TVBX_IPL];
     if (a.tavg_period > 0) tvbx_monitor_obs(a, i, pb, lp, x, y_mon);
     float c[TVBX_IPL][TVBX_C];
-    tvbx_coupling(a, cd, wt, rw, ring, nullptr, i, t, pb, n, lp, c);
+#if TVBX_EMAJ
+    if (emaj) {
+      // ce[k] without dynamic register indexing
+#pragma unroll
+      for (int q = 0; q < TVBX_IPL; ++q)
+#pragma unroll
+        for (int kk = 0; kk < TVBX_C; ++kk) {
+          float v = ce[0][q][kk];
+#pragma unroll
+          for (int k2 = 1; k2 < TVBX_EMAJ; ++k2)
+            v = k == k2 ? ce[k2][q][kk] : v;
+          c[q][kk] = v;
+        }
+    } else
+#endif
+    if (a.csum != nullptr)
+      tvbx_coupling_from_sums(a, i, pb, n, k, lp, c);
+    else
+      tvbx_coupling(a, cd, wt, rw, ring, nullptr, i, t, pb, n, lp, c);
     float yo[TVBX_R][TVBX_IPL];
     tvbx_advance(a, i, pb, n, k, lp, x, c, yo, bad, a.force != nullptr);
 #pragma unroll
@@ -1147,6 +1298,129 @@ tvbx_step_batch(const tvbx_batch_args B) {
     }
   }
 }
+
+#if TVBX_RING_T == 1 || TVBX_RING_T == 2
+#if TVBX_RING_PRE
+// Delay-binned coupling sums on the matrix cores: for a block of 16 target
+// nodes and 64 TVBX_IPL instances, over the launch's n_steps steps,
+//   sum_k[i][p] = sum over the block's (delay d, source j) pairs q of
+//                 W[i][q] * ring[t + k - d][j][p],
+// the sum over delay bins and sources of W_d X(t - d) with X the ring of
+// pre(y) (bfloat16 / float16).  Each pair chunk is, per step and wave, a
+// 16 x 32 by 32 x 16 product (v_mfma_f32_16x16x32_{bf16,f16}, float32
+// accumulation) for each of TVBX_IPL instance offsets: A = the block's
+// weights (exact as high + low part, two MFMAs), B = the ring values of the
+// chunk's 32 pairs.  Lane layouts (checked on the device with integer
+// matrices): A[m][k] from element k % 8 of lane 16 (k / 8) + m, B[k][n]
+// from element k % 8 of lane 16 (k / 8) + n, D[m][n] in element m % 4 of
+// lane 16 (m / 4) + n.  Column n of product j is instance
+// base + TVBX_IPL n + j, so one load of TVBX_IPL adjacent instances fills
+// element e of all TVBX_IPL B operands.  The block's weight tiles and pairs
+// are read once per launch and serve all its steps; every delay is
+// >= n_steps - 1, so all reads are of slots written before the launch.
+// grid (ceil(N / 16), pitch / (64 TVBX_IPL)), block 256: wave w takes
+// instances base = 64 TVBX_IPL x + 16 TVBX_IPL w and the 16 TVBX_IPL after.
+#define TVBX_MFMA_LMAX (TVBX_IPL == 4 ? 8 : 16)
+#if TVBX_RING_T == 1
+typedef __bf16 tvbx_m8 __attribute__((ext_vector_type(8)));
+#define TVBX_MFMA(A, B, C) __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, C, 0, 0, 0)
+#else
+typedef _Float16 tvbx_m8 __attribute__((ext_vector_type(8)));
+#define TVBX_MFMA(A, B, C) __builtin_amdgcn_mfma_f32_16x16x32_f16(A, B, C, 0, 0, 0)
+#endif
+typedef float tvbx_f4 __attribute__((ext_vector_type(4)));
+typedef unsigned short tvbx_u8v __attribute__((ext_vector_type(8)));
+
+extern "C" __global__ void __launch_bounds__(256)
+tvbx_mfma_sums(const tvbx_step_args a) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int blk = blockIdx.x;
+  const int g = lane >> 4;                      // K group: pairs 8 g .. 8 g + 7
+  const int p = (blockIdx.y * 64 + 16 * wave + (lane & 15)) * TVBX_IPL;
+  const int N = a.n_node, H = a.horizon;
+  const int L = min(a.n_steps, TVBX_MFMA_LMAX);
+  long long n0;
+  int t;
+  tvbx_start(a, n0, t);
+  const int c0 = a.mfma_blk[blk], c1 = a.mfma_blk[blk + 1];
+  const __amdgpu_buffer_rsrc_t rs = tvbx_ring_rsrc(a.ring);
+  const unsigned rowb = (unsigned)a.pitch * 2u;
+  const unsigned lane_off = (unsigned)p * 2u;
+  const size_t NP = (size_t)N * a.pitch;
+#pragma unroll
+  for (int kk = 0; kk < TVBX_C; ++kk) {
+    tvbx_f4 acc[TVBX_MFMA_LMAX][TVBX_IPL];
+#pragma unroll
+    for (int k = 0; k < TVBX_MFMA_LMAX; ++k)
+#pragma unroll
+      for (int j = 0; j < TVBX_IPL; ++j) acc[k][j] = tvbx_f4{0.f, 0.f, 0.f, 0.f};
+    for (int ch = c0; ch < c1; ++ch) {
+      const tvbx_u8v *wv =
+          reinterpret_cast<const tvbx_u8v *>(a.mfma_w) + (size_t)ch * 128;
+      const tvbx_m8 whi = __builtin_bit_cast(tvbx_m8, wv[lane]);
+      const tvbx_m8 wlo = __builtin_bit_cast(tvbx_m8, wv[64 + lane]);
+      // this lane's 8 pairs (delay << 16 | column)
+      unsigned pw[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) pw[e] = a.mfma_pairs[(size_t)ch * 32 + 8 * g + e];
+#pragma unroll
+      for (int k = 0; k < TVBX_MFMA_LMAX; ++k) {
+        if (k < L) {
+          tvbx_raw v[8];
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            int slot = t + k - (int)(pw[e] >> 16);
+            slot += slot < 0 ? H : 0;
+            slot -= slot >= H ? H : 0;
+            const unsigned row =
+                ((unsigned)slot * TVBX_R + (unsigned)kk) * (unsigned)N +
+                (pw[e] & 0xffffu);
+            v[e] = tvbx_bload(rs, row * rowb + lane_off, 0u);
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int j = 0; j < TVBX_IPL; ++j) {
+            tvbx_u8v bv;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+#if TVBX_IPL == 4
+              const unsigned w32 = j < 2 ? v[e].x : v[e].y;
+#else
+              const unsigned w32 = (unsigned)v[e];
+#endif
+              bv[e] = (unsigned short)(w32 >> (16 * (j & 1)));
+            }
+            const tvbx_m8 b = __builtin_bit_cast(tvbx_m8, bv);
+            acc[k][j] = TVBX_MFMA(whi, b, acc[k][j]);
+            acc[k][j] = TVBX_MFMA(wlo, b, acc[k][j]);
+          }
+        }
+      }
+    }
+    // D[m][n] of product j: element r of lane 16 (m / 4) + n holds row
+    // m = 4 g + r, instance p + j
+#pragma unroll
+    for (int k = 0; k < TVBX_MFMA_LMAX; ++k) {
+      if (k < L) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int tgt = 16 * blk + 4 * g + r;
+          if (tgt < N) {
+            float o[TVBX_IPL];
+#pragma unroll
+            for (int j = 0; j < TVBX_IPL; ++j) o[j] = acc[k][j][r] * a.mfma_unscale;
+            tvbx_stf(a.csum + ((size_t)k * TVBX_C + kk) * NP +
+                         (size_t)tgt * a.pitch + p, o);
+          }
+        }
+      }
+    }
+  }
+}
+#endif
+#endif
 #endif
 
 // Fill the ring from the current state: slot t = h(x); other slots 0

@@ -107,11 +107,26 @@ typedef struct tvbx_step_args {
   int weight_groups;
   /* forced (teacher) steps: when non-null, x(n + 1) is read from
    * force[(n + 1 - force_step0)][S][N][pitch] instead of being drawn, and
-   * logp[N][pitch] accumulates the Euler-Maruyama transition log density
-   * -(x(n+1) - x(n) - dt f)^2 / (2 dt g^2) of each (node, instance) */
+   * logp[2][N][pitch] accumulates the Euler-Maruyama transition log density
+   * -(x(n+1) - x(n) - dt f)^2 / (2 dt g^2) of each (node, instance) as a
+   * compensated float32 sum: plane 0 the running sum, plane 1 the rounding
+   * error carried along (Neumaier), the value being their sum */
   const float *force;
   long long force_step0;
-  double *logp;
+  float *logp;
+  /* Delay-binned coupling on the matrix cores (bfloat16 / float16 rings
+   * holding pre(y)): targets in blocks of 16; block b's (delay, column)
+   * pairs, in chunks of 32, are chunks [mfma_blk[b], mfma_blk[b + 1]); per
+   * chunk mfma_pairs holds the 32 packed pairs (delay << 16 | column) and
+   * mfma_w the block's 16 x 32 weights as two MFMA A operands (high and low
+   * part, 64 lanes x 8 elements each), scaled by 1 / mfma_unscale.
+   * csum [n_steps][C][N][pitch]: coupling sums of the launch's steps, which
+   * the blocked kernel reads instead of gathering when non-null. */
+  const int *mfma_blk;
+  const unsigned *mfma_pairs;
+  const unsigned short *mfma_w;
+  float mfma_unscale;
+  float *csum;
 } tvbx_step_args;
 
 /* Argument block of the batched step kernel: per subject (one connectome
@@ -160,8 +175,11 @@ typedef struct tvbx_adj_args {
 typedef struct tvbx_logp_args {
   tvbx_step_args f;
   const float *rec_sum;   /* [n_steps][C][N][pitch] */
-  double *g_c;            /* [n_gc][N][pitch] += d logp / d coupling param */
-  double *g_m;            /* [n_gm][N][pitch] += d logp / d model param */
+  /* [2][n][N][pitch] compensated float32 sums (plane 0 the sum, plane 1
+   * the carried rounding error) += d logp / d coupling / model param */
+  float *g_c;
+  float *g_m;
+  int n_gc, n_gm;         /* parameters in each (plane stride n N pitch) */
 } tvbx_logp_args;
 
 #ifdef __cplusplus

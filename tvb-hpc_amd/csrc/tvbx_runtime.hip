@@ -150,6 +150,7 @@ struct tvbx_module {
   hipFunction_t blocked = nullptr;  // absent in the reference ordering
   hipFunction_t batch = nullptr;    // absent in the reference ordering
   hipFunction_t logp_grad = nullptr;  // present when emitted with the adjoint
+  hipFunction_t mfma = nullptr;  // bfloat16 / float16 rings holding pre(y)
   std::vector<char> image;
 };
 
@@ -211,7 +212,16 @@ struct tvbx_sim {
   bool ck_recorded = false;
   DevBuf lam, mu, rec_x, rec_sum, forcing, g_c, g_m, g_out, save_state,
       save_ring;
+  // delay-binned coupling on the matrix cores (tvbx_sim_set_coupling)
+  bool mfma = false;
+  DevBuf mfma_blk, mfma_pairs, mfma_w, csum;
+  float mfma_unscale = 1.0f;
+  int mfma_chunks = 0;
 };
+
+// steps per launch of tvbx_mfma_sums (its accumulators: steps x ipl tiles)
+#define TVBX_MFMA_LMAX(ipl) ((ipl) == 4 ? 8 : 16)
+static int mfma_tables(tvbx_sim *s);
 
 // the forward step kernel of the simulation's launch shape
 static hipFunction_t step_fn(const tvbx_sim *s) {
@@ -324,6 +334,8 @@ static int module_from_image(tvbx_ctx *c, const void *image, size_t size,
   if (hipModuleGetFunction(&m->batch, m->mod, "tvbx_step_batch") !=
       hipSuccess)
     m->batch = nullptr;
+  if (hipModuleGetFunction(&m->mfma, m->mod, "tvbx_mfma_sums") != hipSuccess)
+    m->mfma = nullptr;
   (void)hipGetLastError();
   *out = m;
   return TVBX_OK;
@@ -790,7 +802,7 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
   if (s->forced_run) {
     a.force = s->force_ext ? s->force_ext : (const float *)s->force.p;
     a.force_step0 = s->force_step0;
-    a.logp = (double *)s->logp.p;
+    a.logp = (float *)s->logp.p;
     if (s->forced_rec) {
       a.rec_sum = (float *)s->rec_sum.p;
       a.rec_step0 = s->force_step0;
@@ -799,6 +811,35 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
   a.ring_unscale = d.ring_type == 0   ? 1.0f
                    : d.ring_type == 3 ? 1.0f / (255.0f * d.ring_scale)
                                       : 1.0f / d.ring_scale;
+  if (s->mfma) {
+    a.mfma_blk = (const int *)s->mfma_blk.p;
+    a.mfma_pairs = (const unsigned *)s->mfma_pairs.p;
+    a.mfma_w = (const unsigned short *)s->mfma_w.p;
+    a.mfma_unscale = s->mfma_unscale;
+    a.csum = (float *)s->csum.p;
+  }
+}
+
+// One forward launch of a.n_steps steps: the step kernel of the launch
+// shape, or with the matrix-core coupling the sums kernel and then the
+// blocked kernel reading its sums.
+static hipError_t launch_steps(tvbx_sim *s, tvbx_step_args &a,
+                               hipStream_t st) {
+  const tvbx_sim_desc &d = s->d;
+  const unsigned gx = s->pitch / (64 * d.ipl);
+  const unsigned gy = (d.n_node + d.node_block - 1) / d.node_block;
+  const unsigned bx = 64u * d.waves;
+  void *params[] = {&a};
+  if (s->mfma) {
+    hipError_t e = hipModuleLaunchKernel(
+        s->mod->mfma, (d.n_node + 15) / 16, s->pitch / (64 * d.ipl), 1, 256,
+        1, 1, 0, st, params, nullptr);
+    if (e != hipSuccess) return e;
+    return hipModuleLaunchKernel(s->mod->blocked, gx, gy, 1, bx, 1, 1, 0, st,
+                                 params, nullptr);
+  }
+  return hipModuleLaunchKernel(step_fn(s), gx, gy, 1, bx, 1, 1, 0, st, params,
+                               nullptr);
 }
 
 // reference ordering: the stale block read by step `step`
@@ -944,6 +985,8 @@ extern "C" int tvbx_sim_destroy(tvbx_sim *s) {
                    &s->rec_sum, &s->forcing, &s->g_c, &s->g_m, &s->g_out,
                    &s->save_state, &s->save_ring};
   for (DevBuf *b : adj) dfree(*b);
+  DevBuf *mf[] = {&s->mfma_blk, &s->mfma_pairs, &s->mfma_w, &s->csum};
+  for (DevBuf *b : mf) dfree(*b);
   for (auto &b : s->ck_state) dfree(b);
   for (auto &b : s->ck_ring) dfree(b);
   for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
@@ -1022,6 +1065,137 @@ extern "C" int tvbx_sim_set_connectome(tvbx_sim *s, const int32_t *row,
   s->h_w.assign(weight, weight + nw);
   s->h_delays.assign(delays, delays + (size_t)d.n_group * d.nnz);
   dfree(s->ccol);
+  if (s->mfma) return mfma_tables(s);
+  return TVBX_OK;
+}
+
+// ---------------------------------------------- matrix-core coupling
+// Host tables of tvbx_mfma_sums from the CSR (group 0): per block of 16
+// targets the distinct (delay, column) pairs its edges read, sorted, padded
+// with weight-0 copies of the last pair to chunks of 32; per chunk the 32
+// packed pairs and the 16 x 32 weight tile in the MFMA A-operand lane layout
+// (A[m][k] in element k % 8 of lane 16 (k / 8) + m), as a high part
+// (weight rounded to the ring's 16-bit type) and a low part (the rounding
+// remainder, rounded again), after scaling by a power of two that brings
+// the largest weight into [1, 2).
+static unsigned short mfma_bits(float v, int ring_type) {
+  if (ring_type == 2) {
+    const _Float16 h = (_Float16)v;
+    unsigned short b;
+    memcpy(&b, &h, 2);
+    return b;
+  }
+  unsigned u;
+  memcpy(&u, &v, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (unsigned short)(u >> 16);
+}
+
+static float mfma_value(unsigned short b, int ring_type) {
+  if (ring_type == 2) {
+    _Float16 h;
+    memcpy(&h, &b, 2);
+    return (float)h;
+  }
+  const unsigned u = (unsigned)b << 16;
+  float v;
+  memcpy(&v, &u, 4);
+  return v;
+}
+
+static int mfma_tables(tvbx_sim *s) {
+  const tvbx_sim_desc &d = s->d;
+  const int N = d.n_node, nb = (N + 15) / 16;
+  float wmax = 0.0f;
+  for (int e = 0; e < d.nnz; ++e) wmax = std::max(wmax, std::fabs(s->h_w[e]));
+  const int ex = wmax > 0.0f ? (int)std::floor(std::log2(wmax)) : 0;
+  const float wscale = std::ldexp(1.0f, -ex);
+  std::vector<int> blk(nb + 1, 0);
+  std::vector<unsigned> pairs;
+  std::vector<unsigned short> w;
+  for (int b = 0; b < nb; ++b) {
+    std::vector<unsigned> keys;
+    for (int i = 16 * b; i < std::min(N, 16 * b + 16); ++i)
+      for (int e = s->h_row[i]; e < s->h_row[i + 1]; ++e)
+        keys.push_back(((unsigned)s->h_delays[e] << 16) |
+                       (unsigned)s->h_col[e]);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const size_t U = keys.size(), nch = (U + 31) / 32;
+    std::vector<float> A(16 * nch * 32, 0.0f);  // [m][k]
+    for (int i = 16 * b; i < std::min(N, 16 * b + 16); ++i)
+      for (int e = s->h_row[i]; e < s->h_row[i + 1]; ++e) {
+        const unsigned key = ((unsigned)s->h_delays[e] << 16) |
+                             (unsigned)s->h_col[e];
+        const size_t k =
+            std::lower_bound(keys.begin(), keys.end(), key) - keys.begin();
+        A[(size_t)(i - 16 * b) * nch * 32 + k] += s->h_w[e] * wscale;
+      }
+    for (size_t c = 0; c < nch; ++c) {
+      for (int k = 0; k < 32; ++k)
+        pairs.push_back(keys[std::min(c * 32 + k, U - 1)]);
+      std::vector<unsigned short> hi(512), lo(512);
+      for (int l = 0; l < 64; ++l)
+        for (int el = 0; el < 8; ++el) {
+          const int m = l % 16, k = 8 * (l / 16) + el;
+          const float v = A[(size_t)m * nch * 32 + c * 32 + k];
+          const unsigned short h = mfma_bits(v, d.ring_type);
+          hi[l * 8 + el] = h;
+          lo[l * 8 + el] =
+              mfma_bits(v - mfma_value(h, d.ring_type), d.ring_type);
+        }
+      w.insert(w.end(), hi.begin(), hi.end());
+      w.insert(w.end(), lo.begin(), lo.end());
+    }
+    blk[b + 1] = blk[b] + (int)nch;
+  }
+  int rc = dalloc(s->mfma_blk, blk.size() * 4);
+  if (rc == TVBX_OK) rc = dalloc(s->mfma_pairs, std::max<size_t>(pairs.size(), 1) * 4);
+  if (rc == TVBX_OK) rc = dalloc(s->mfma_w, std::max<size_t>(w.size(), 1) * 2);
+  if (rc == TVBX_OK)
+    rc = dalloc(s->csum, (size_t)d.steps_per_launch * d.n_coupled * s->NP * 4);
+  if (rc != TVBX_OK) return rc;
+  HIPCHK(hipMemcpy(s->mfma_blk.p, blk.data(), blk.size() * 4,
+                   hipMemcpyHostToDevice));
+  if (!pairs.empty()) {
+    HIPCHK(hipMemcpy(s->mfma_pairs.p, pairs.data(), pairs.size() * 4,
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->mfma_w.p, w.data(), w.size() * 2,
+                     hipMemcpyHostToDevice));
+  }
+  s->mfma_unscale = std::ldexp(1.0f, ex);
+  s->mfma_chunks = blk[nb];
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_sim_set_coupling(tvbx_sim *s, int mode) {
+  CHECK(s && (mode == 0 || mode == 1), "bad argument");
+  const tvbx_sim_desc &d = s->d;
+  HIPCHK(hipSetDevice(s->ctx->device));
+  if (mode == 0) {
+    if (s->mfma) drop_graphs(s);
+    s->mfma = false;
+    return TVBX_OK;
+  }
+  CHECK(s->mod->mfma, "the module has no matrix-core coupling kernel (it "
+                      "needs a bfloat16 or float16 ring holding pre(y))");
+  CHECK(d.order == 0, "matrix-core coupling needs the CUDA ordering");
+  CHECK(d.node_block == d.waves && d.waves <= 8 && d.node_block < d.n_node,
+        "matrix-core coupling advances with the blocked kernel: node_block "
+        "== waves <= 8 and node_block < n_node");
+  CHECK(d.steps_per_launch <= TVBX_MFMA_LMAX(d.ipl),
+        "matrix-core coupling takes at most %d steps per launch with %d "
+        "instances per lane", TVBX_MFMA_LMAX(d.ipl), d.ipl);
+  CHECK(d.n_group == 1 && !d.weight_groups,
+        "matrix-core coupling takes one delay and weight row");
+  CHECK(!d.compat_outputs, "matrix-core coupling keeps no compat outputs");
+  CHECK((double)d.horizon * d.n_ring * s->NP * s->esz < 4294967296.0,
+        "matrix-core coupling needs a ring under 4 GiB");
+  CHECK(s->have_connectome, "set the connectome first");
+  int rc = mfma_tables(s);
+  if (rc != TVBX_OK) return rc;
+  drop_graphs(s);
+  s->mfma = true;
   return TVBX_OK;
 }
 
@@ -1312,9 +1486,7 @@ static int capture_window(tvbx_sim *s, long long W, hipGraphExec_t *out) {
     a.slot0 = (int)(k % d.horizon);
     a.n_steps = d.steps_per_launch;
     a.clock = (const long long *)s->clock.p;
-    void *params[] = {&a};
-    e = hipModuleLaunchKernel(step_fn(s), gx, gy, 1, bx, 1, 1, 0, st,
-                              params, nullptr);
+    e = launch_steps(s, a, st);
   }
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st,
@@ -1438,15 +1610,13 @@ extern "C" int tvbx_sim_run(tvbx_sim *s, long long n_steps,
       a.noise = (const float *)s->noise.p + (size_t)done * (tstride ? SNP : 0);
       a.noise_tstride = tstride ? (long long)SNP : 0;
     }
-    void *params[] = {&a};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (s->profile) {
       e0 = next_event(s);
       e1 = next_event(s);
       if (e0) hipEventRecord(e0, s->ctx->stream);
     }
-    HIPCHK(hipModuleLaunchKernel(step_fn(s), gx, gy, 1, bx, 1, 1, 0,
-                                 s->ctx->stream, params, nullptr));
+    HIPCHK(launch_steps(s, a, s->ctx->stream));
     if (s->profile && e1) hipEventRecord(e1, s->ctx->stream);
     s->launches++;
     if (d.tavg_period > 0) {
@@ -1593,18 +1763,35 @@ extern "C" int tvbx_sim_fc_sc_error(tvbx_sim *s, const float *sc, int p0,
 }
 
 // out[p] = sum over nodes i (ascending) of part[i][p].  grid pitch / 64.
-__global__ void k_logp_sum(const double *__restrict__ part, int N, int pitch,
-                           double *__restrict__ out) {
+// s + c += x, compensated (Neumaier); operations as written
+__device__ __forceinline__ void csum_add(float &s, float &c, float x) {
+  const float t = s + x;
+  c += fabsf(s) >= fabsf(x) ? (s - t) + x : (x - t) + s;
+  s = t;
+}
+
+// per instance: the compensated sum over nodes (ascending) of the
+// per-(node, instance) compensated sums part [2][N][pitch] -> out [2][pitch]
+__global__ void k_logp_sum(const float *__restrict__ part, int N, int pitch,
+                           float *__restrict__ out) {
   const int p = blockIdx.x * 64 + threadIdx.x;
-  double acc = 0.0;
-  for (int i = 0; i < N; ++i) acc += part[(size_t)i * pitch + p];
-  out[p] = acc;
+  const size_t NP = (size_t)N * pitch;
+  float s = 0.0f, c = 0.0f;
+  for (int i = 0; i < N; ++i) {
+    csum_add(s, c, part[(size_t)i * pitch + p]);
+    c += part[NP + (size_t)i * pitch + p];
+  }
+  out[p] = s;
+  out[pitch + p] = c;
 }
 
 extern "C" int tvbx_sim_logp(tvbx_sim *s, const float *traj,
                              long long n_steps, int window, double *logp,
                              int n_gc, float *g_coupling, int n_gm,
                              float *g_model) {
+  CHECK(s && !s->mfma, "switch the matrix-core coupling off first "
+                       "(tvbx_sim_set_coupling(s, 0)): the gradient and "
+                       "likelihood paths gather");
   CHECK(s && traj && logp, "null argument");
   CHECK(n_steps >= 1, "need at least one transition");
   CHECK(s->have_connectome, "set the connectome first");
@@ -1628,16 +1815,17 @@ extern "C" int tvbx_sim_logp(tvbx_sim *s, const float *traj,
   int rc = dalloc(s->force, d.n_inst == s->pitch && on_device(traj)
                                 ? 4
                                 : (size_t)(W + 1) * SNP * 4);
-  if (rc == TVBX_OK) rc = dalloc(s->logp, s->NP * 8);
-  if (rc == TVBX_OK) rc = dalloc(s->logp_sum, (size_t)s->pitch * 8);
+  // compensated float32 sums: two planes (sum, carried rounding error)
+  if (rc == TVBX_OK) rc = dalloc(s->logp, 2 * s->NP * 4);
+  if (rc == TVBX_OK) rc = dalloc(s->logp_sum, 2 * (size_t)s->pitch * 4);
   if (rc == TVBX_OK) rc = fill_f32(s, s->force, 0.0f);
-  if (rc == TVBX_OK) rc = fill_f64(s, s->logp, 0.0);
+  if (rc == TVBX_OK) rc = fill_f32(s, s->logp, 0.0f);
   if (grads) {
     if (rc == TVBX_OK) rc = dalloc(s->rec_sum, (size_t)W * d.n_coupled * s->NP * 4);
-    if (rc == TVBX_OK) rc = dalloc(s->lg_c, (size_t)ngc * s->NP * 8);
-    if (rc == TVBX_OK) rc = dalloc(s->lg_m, (size_t)ngm * s->NP * 8);
-    if (rc == TVBX_OK) rc = fill_f64(s, s->lg_c, 0.0);
-    if (rc == TVBX_OK) rc = fill_f64(s, s->lg_m, 0.0);
+    if (rc == TVBX_OK) rc = dalloc(s->lg_c, 2 * (size_t)ngc * s->NP * 4);
+    if (rc == TVBX_OK) rc = dalloc(s->lg_m, 2 * (size_t)ngm * s->NP * 4);
+    if (rc == TVBX_OK) rc = fill_f32(s, s->lg_c, 0.0f);
+    if (rc == TVBX_OK) rc = fill_f32(s, s->lg_m, 0.0f);
   }
   if (rc == TVBX_OK) rc = tvbx_sim_set_state(s, traj);
   // frames already in device memory with the pitched layout are read in
@@ -1667,8 +1855,10 @@ extern "C" int tvbx_sim_logp(tvbx_sim *s, const float *traj,
     L.f.force = fr;
     L.f.n_steps = (int)w;
     L.rec_sum = (const float *)s->rec_sum.p;
-    L.g_c = (double *)s->lg_c.p;
-    L.g_m = (double *)s->lg_m.p;
+    L.g_c = (float *)s->lg_c.p;
+    L.g_m = (float *)s->lg_m.p;
+    L.n_gc = ngc;
+    L.n_gm = ngm;
     void *params[] = {&L};
     const hipError_t e = hipModuleLaunchKernel(
         s->mod->logp_grad, s->pitch / (64 * d.ipl), d.n_node, 1, 64, 1, 1, 0,
@@ -1678,35 +1868,43 @@ extern "C" int tvbx_sim_logp(tvbx_sim *s, const float *traj,
   }
   if (rc != TVBX_OK) return rc;
   hipLaunchKernelGGL(k_logp_sum, dim3(s->pitch / 64), dim3(64), 0,
-                     s->ctx->stream, (const double *)s->logp.p, d.n_node,
-                     s->pitch, (double *)s->logp_sum.p);
+                     s->ctx->stream, (const float *)s->logp.p, d.n_node,
+                     s->pitch, (float *)s->logp_sum.p);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(logp, s->logp_sum.p, (size_t)d.n_inst * 8,
+  std::vector<float> hl(2 * (size_t)s->pitch), hc, hm;
+  HIPCHK(hipMemcpyAsync(hl.data(), s->logp_sum.p, hl.size() * 4,
                         hipMemcpyDeviceToHost, s->ctx->stream));
-  std::vector<double> hc, hm;
   if (grads) {
-    hc.resize((size_t)ngc * s->NP);
-    hm.resize((size_t)ngm * s->NP);
-    HIPCHK(hipMemcpyAsync(hc.data(), s->lg_c.p, hc.size() * 8,
+    hc.resize(2 * (size_t)ngc * s->NP);
+    hm.resize(2 * (size_t)ngm * s->NP);
+    HIPCHK(hipMemcpyAsync(hc.data(), s->lg_c.p, hc.size() * 4,
                           hipMemcpyDeviceToHost, s->ctx->stream));
-    HIPCHK(hipMemcpyAsync(hm.data(), s->lg_m.p, hm.size() * 8,
+    HIPCHK(hipMemcpyAsync(hm.data(), s->lg_m.p, hm.size() * 4,
                           hipMemcpyDeviceToHost, s->ctx->stream));
   }
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
   const int N = d.n_node, P = d.n_inst;
+  // a compensated sum's value: the sum plane plus the error plane
+  for (int p = 0; p < P; ++p)
+    logp[p] = (double)hl[p] + (double)hl[s->pitch + p];
+  const size_t cplane = (size_t)ngc * s->NP, mplane = (size_t)ngm * s->NP;
   // coupling parameters (kind 'inst'): summed over nodes in ascending order
   for (int g = 0; g < n_gc; ++g)
     for (int p = 0; p < P; ++p) {
       double acc = 0.0;
-      for (int i = 0; i < N; ++i)
-        acc += hc[((size_t)g * N + i) * s->pitch + p];
+      for (int i = 0; i < N; ++i) {
+        const size_t k = ((size_t)g * N + i) * s->pitch + p;
+        acc += (double)hc[k] + (double)hc[cplane + k];
+      }
       g_coupling[(size_t)g * P + p] = (float)acc;
     }
   for (int g = 0; g < n_gm; ++g)
     for (int i = 0; i < N; ++i)
-      for (int p = 0; p < P; ++p)
+      for (int p = 0; p < P; ++p) {
+        const size_t k = ((size_t)g * N + i) * s->pitch + p;
         g_model[((size_t)g * N + i) * P + p] =
-            (float)hm[((size_t)g * N + i) * s->pitch + p];
+            (float)((double)hm[k] + (double)hm[mplane + k]);
+      }
   return TVBX_OK;
 }
 
@@ -2057,6 +2255,9 @@ static int build_csc(tvbx_sim *s) {
 
 extern "C" int tvbx_sim_checkpoint_run(tvbx_sim *s, long long n_steps,
                                        int window) {
+  CHECK(s && !s->mfma, "switch the matrix-core coupling off first "
+                       "(tvbx_sim_set_coupling(s, 0)): the gradient and "
+                       "likelihood paths gather");
   CHECK(s && n_steps > 0 && window >= 0, "bad argument");
   CHECK(window == 0 || n_steps % window == 0,
         "n_steps must be a multiple of window");
@@ -2115,6 +2316,9 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
                                 long long n_samples, int n_gc,
                                 float *g_coupling, int n_gm, float *g_model,
                                 int out_on_device) {
+  CHECK(s && !s->mfma, "switch the matrix-core coupling off first "
+                       "(tvbx_sim_set_coupling(s, 0)): the gradient and "
+                       "likelihood paths gather");
   CHECK(s, "null sim");
   int rc = adjoint_checks(s);
   if (rc != TVBX_OK) return rc;

@@ -66,6 +66,15 @@ def valid_ipls(cfg):
 def candidates(cfg):
     N = cfg.csr.nnode
     out = []
+    if getattr(cfg, 'coupling', 'gather') == 'mfma':
+        # blocked advance shapes only, at most mfma_steps(ipl) steps
+        for ipl in valid_ipls(cfg):
+            for spl in sorted({min(b, mfma_steps(ipl))
+                               for b in block_lengths(cfg)}):
+                for waves in (1, 2, 4, 8):
+                    out.append(dict(ipl=ipl, waves=waves, node_block=waves,
+                                    steps_per_launch=spl))
+        return out
     for ipl in valid_ipls(cfg):
         for waves, nb in ((4, 4), (2, 2), (8, 8), (4, 8), (8, 16), (1, 1),
                           (16, 16)):
@@ -82,6 +91,11 @@ def candidates(cfg):
                     out.append(dict(ipl=ipl, waves=waves, node_block=waves,
                                     steps_per_launch=spl))
     return out
+
+
+def mfma_steps(ipl):
+    "steps per launch the matrix-core sums kernel holds accumulators for"
+    return 8 if ipl == 4 else 16
 
 
 def block_lengths(cfg):
@@ -103,6 +117,11 @@ def default_launch(cfg):
     ipl = 4 if 4 in ipls and cfg.n_inst >= 1024 else max(
         [i for i in ipls if i <= 2] or [1])
     bl = block_lengths(cfg)
+    if getattr(cfg, 'coupling', 'gather') == 'mfma':
+        # the sums come from the matrix-core kernel: blocks of up to
+        # mfma_steps(ipl) steps, a light advance kernel
+        return dict(ipl=ipl, waves=4, node_block=4,
+                    steps_per_launch=min(max(bl or [1]), mfma_steps(ipl)))
     if bl and max(bl) >= 8:
         la = dict(ipl=ipl, waves=1, node_block=1,
                   steps_per_launch=min(max(bl), 32), min_waves=4)
@@ -164,6 +183,11 @@ def _time(cfg, cand, x0, steps, warmup, ctx):
     return ms
 
 
+def _emaj_allowed(cfg):
+    "edge-major sums need a pre() that ignores post_syn (the CUDA order)"
+    return not cfg.cfun.pre_uses_post_syn()
+
+
 def tune(cfg, steps=100, warmup=10, ctx=None, log=None,
          variants=(dict(unroll=16), dict(unroll=4), dict(min_waves=4),
                    dict(min_waves=4, unroll=4), dict(min_waves=4, unroll=16),
@@ -196,13 +220,20 @@ def tune(cfg, steps=100, warmup=10, ctx=None, log=None,
     bases = []
     for ipl in sorted({c['ipl'] for _, c in results}):
         bases += [c for _, c in results if c['ipl'] == ipl][:top]
+    blocked = [b for b in bases if b['steps_per_launch'] > 1 and
+               b['node_block'] < cfg.csr.nnode]
+    emaj_ok = _emaj_allowed(cfg)
     for base in bases:
-        for var in variants:
+        extra = []
+        if emaj_ok and base in blocked and base['steps_per_launch'] <= 16:
+            e = min(v for v in (4, 8, 16) if v >= base['steps_per_launch'])
+            extra = [dict(emaj=e), dict(emaj=e, xcd_map=True),
+                     dict(emaj=e, min_waves=4), dict(emaj=e, unroll=8),
+                     dict(emaj=e, xcd_map=True, min_waves=4)]
+        for var in list(variants) + extra:
             if all(base.get(k) == v for k, v in var.items()):
                 continue
-            if 'min_waves' in var and not (base['steps_per_launch'] > 1 and
-                                           base['node_block'] <
-                                           cfg.csr.nnode):
+            if 'min_waves' in var and base not in blocked:
                 continue   # only the blocked kernel takes the bound
             cand = dict(base, **var)
             ms = _time(cfg, cand, x0, steps, warmup, ctx)

@@ -53,6 +53,9 @@ class SweepConfig:
     # monitor samples buffered on the device between FC merges (each merge
     # reads and writes the FP64 co-moment triangle once)
     sample_slots: int = 256
+    # coupling sums: 'gather' (CSR rows) or 'mfma' (delay-binned, matrix
+    # cores)
+    coupling: str = 'gather'
 
     @property
     def horizon(self):
@@ -81,7 +84,7 @@ class SweepConfig:
                            dict(self.kinds), gs, self.tavg_period, self.bold,
                            self.fc, self.bold_obs, self.seed,
                            self.description, self.ring_dtype, self.ring_scale,
-                           self.delay_bins)
+                           self.delay_bins, self.sample_slots, self.coupling)
 
     def initial_state(self, dtype=np.float32):
         rng = np.random.default_rng(1000 + self.seed)
@@ -94,7 +97,7 @@ class SweepConfig:
                     group_size=self.group_size, tavg_period=self.tavg_period,
                     bold=self.bold, fc=self.fc, bold_obs=self.bold_obs,
                     ring_dtype=self.ring_dtype, ring_scale=self.ring_scale,
-                    sample_slots=self.sample_slots)
+                    sample_slots=self.sample_slots, coupling=self.coupling)
         opts.update(kw)
         return Simulation(self.model, self.cfun, self.scheme, self.csr,
                           self.n_inst, **opts)
@@ -247,11 +250,12 @@ def jansenrit998(n_inst=1024, n_steps=10000, tavg_period=20):
 
 
 def jansenrit998_mixed(n_inst=1024, n_steps=10000, tavg_period=20,
-                       ring_dtype='bf16', bins=32):
+                       ring_dtype='bf16', bins=32, coupling='gather'):
     """Jansen-Rit N=998 with delays quantised to `bins` values and a
     reduced-precision history ring (bf16 / f16, or 8-bit fixed point) holding
     1/cmax-scaled sigmoid outputs in [0, 1]; state, sums and FC stay
-    float32 / float64."""
+    float32 / float64.  coupling 'mfma': the sum over bins of W_d X(t - d)
+    on the matrix cores (bf16 / f16 rings)."""
     cfg = jansenrit998(n_inst, n_steps, tavg_period)
     csr, _ = cfg.csr.binned(bins)
     cfg.csr = csr.sorted_by_delay()
@@ -260,6 +264,10 @@ def jansenrit998_mixed(n_inst=1024, n_steps=10000, tavg_period=20,
     cfg.ring_scale = 1.0 / float(cfg.cfun.param['cmax'])
     cfg.delay_bins = bins
     cfg.description += ', %d delay bins, %s ring' % (bins, ring_dtype)
+    if coupling != 'gather':
+        cfg.name += '_' + coupling
+        cfg.coupling = coupling
+        cfg.description += ', %s coupling' % coupling.upper()
     return cfg
 
 
@@ -317,7 +325,11 @@ ALL = {'kuramoto76': kuramoto76, 'g2do84': g2do84,
        'jansenrit998_f16': lambda **kw: jansenrit998_mixed(
            ring_dtype='f16', **kw),
        'jansenrit998_u8': lambda **kw: jansenrit998_mixed(
-           ring_dtype='u8', **kw)}
+           ring_dtype='u8', **kw),
+       'jansenrit998_bf16_mfma': lambda **kw: jansenrit998_mixed(
+           ring_dtype='bf16', coupling='mfma', **kw),
+       'jansenrit998_f16_mfma': lambda **kw: jansenrit998_mixed(
+           ring_dtype='f16', coupling='mfma', **kw)}
 
 # sweeps over subjects with different connectomes (tvb_hpc.hip.Batch)
 BATCHES = {'rww_uneven': rww_uneven}
@@ -336,7 +348,8 @@ def all_specs():
             sp = cfg.spec(ipl=la['ipl'], unroll=la.get('unroll', 8),
                           min_waves=la.get('min_waves', 1),
                           pipe=la.get('pipe', False),
-                          prio=la.get('prio', True))
+                          prio=la.get('prio', True),
+                          emaj=la.get('emaj', 0))
             if sp.key() not in keys:
                 keys.add(sp.key())
                 specs.append(sp)
@@ -347,7 +360,8 @@ def all_specs():
             sp = cfg.spec(ipl=la['ipl'], unroll=la.get('unroll', 8),
                           min_waves=la.get('min_waves', 1),
                           pipe=la.get('pipe', False),
-                          prio=la.get('prio', True))
+                          prio=la.get('prio', True),
+                          emaj=la.get('emaj', 0))
             if sp.key() not in keys:
                 keys.add(sp.key())
                 specs.append(sp)

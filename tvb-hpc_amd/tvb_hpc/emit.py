@@ -96,7 +96,7 @@ class KernelSpec:
     def __init__(self, model: BaseModel, cfun: BaseCoupling, scheme: EulerStep,
                  kinds=None, sigma_kind=None, ring='coupled', ring_pre=None,
                  order='cuda', ipl=1, ring_dtype='f32', adjoint=False,
-                 unroll=8, min_waves=1, pipe=False, prio=True):
+                 unroll=8, min_waves=1, pipe=False, prio=True, emaj=0):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.adjoint = bool(adjoint)
         if unroll not in (4, 8, 16, 32):
@@ -111,6 +111,11 @@ class KernelSpec:
         self.pipe = bool(pipe)
         # wave priority falling with a multi-step launch's progress
         self.prio = bool(prio)
+        # blocked launches of up to `emaj` steps form their coupling sums
+        # first, edges outer and steps inner (0: step by step)
+        if emaj not in (0, 4, 8, 16):
+            raise ValueError('emaj must be 0, 4, 8 or 16')
+        self.emaj = int(emaj)
         if order not in ('cuda', 'reference'):
             raise ValueError('order must be cuda or reference')
         self.order = order
@@ -141,6 +146,9 @@ class KernelSpec:
                              "ring='coupled'")
         self.ring_pre = bool(ring_pre)
         self.uses_post = uses_post and not self.ring_pre
+        if self.emaj and (self.uses_post or order != 'cuda'):
+            raise ValueError('emaj needs the CUDA ordering and a pre() that '
+                             'does not read post_syn')
         if ring_dtype == 'u8' and not (self.ring_pre and
                                        cfun.pre_bounds() is not None):
             raise ValueError('an 8-bit history needs a ring holding a '
@@ -271,6 +279,7 @@ class KernelSpec:
         if self.pipe:
             lines.append('#define TVBX_PIPE 1')
         lines.append('#define TVBX_PRIO %d' % int(self.prio))
+        lines.append('#define TVBX_EMAJ %d' % self.emaj)
         # lane parameters
         fields = self._lane_fields()
         lines.append('struct tvbx_lane_params {')

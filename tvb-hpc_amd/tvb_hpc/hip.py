@@ -101,6 +101,7 @@ _SIGS = {
     'tvbx_sim_set_noise': [_P, _I, ctypes.c_ulonglong],
     'tvbx_sim_run': [_P, _LL, _P, _LL],
     'tvbx_sim_set_exec': [_P, _I, _I, _I],
+    'tvbx_sim_set_coupling': [_P, _I],
     'tvbx_sim_checkpoint_run': [_P, _LL, _I],
     'tvbx_sim_adjoint': [_P, _P, _I, _LL, _LL, _I, _P, _I, _P, _I],
     'tvbx_sim_get_outputs': [_P, _P, _P, _P],
@@ -379,7 +380,9 @@ class Simulation:
         their model / coupling defaults as literals.
     monitor: tavg_period (steps), bold (bool), fc (bool), sample_slots.
     launch: ipl, waves, node_block, steps_per_launch, unroll, min_waves,
-        pipe, prio, xcd_map (see tvb_hpc.autotune).
+        pipe, prio, xcd_map, emaj (see tvb_hpc.autotune); coupling 'gather' (CSR
+        rows) or 'mfma' (delay-binned sums on the matrix cores; bfloat16 /
+        float16 rings holding pre(y), blocked launches of at most 16 steps).
     """
 
     def __init__(self, model, cfun, scheme, csr: CSR, n_inst, horizon=None,
@@ -389,7 +392,7 @@ class Simulation:
                  steps_per_launch=1, compat_outputs=False, dt=None,
                  order='cuda', ipl=1, ring_dtype='f32', ring_scale=1.0,
                  adjoint=False, unroll=8, min_waves=1, pipe=False,
-                 prio=True, xcd_map=False,
+                 prio=True, xcd_map=False, coupling='gather', emaj=0,
                  context=None):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.csr = csr
@@ -414,7 +417,7 @@ class Simulation:
                                     ring_pre=ring_pre, order=order, ipl=ipl,
                                     ring_dtype=ring_dtype, adjoint=adjoint,
                                     unroll=unroll, min_waves=min_waves,
-                                    pipe=pipe, prio=prio)
+                                    pipe=pipe, prio=prio, emaj=emaj)
         if ring_scale != 1.0 and not self.spec.ring_pre:
             raise ValueError('ring_scale needs a ring that holds pre(y)')
         if ring_dtype == 'u8':
@@ -492,6 +495,15 @@ class Simulation:
             self.h, _ptr(csr.row), _ptr(csr.col), _ptr(csr.weight),
             _ptr(csr.delays)))
         self.set_params(resolved)
+        self.coupling = None
+        self.set_coupling(coupling)
+
+    def set_coupling(self, mode):
+        """'gather': CSR row sums; 'mfma': delay-binned sums on the matrix
+        cores (tvbx_sim_set_coupling)."""
+        code = {'gather': 0, 'mfma': 1}[mode]
+        check(lib().tvbx_sim_set_coupling(self.h, code))
+        self.coupling = mode
 
     def __del__(self):
         h = getattr(self, 'h', None)
@@ -797,7 +809,7 @@ class Batch:
     def __init__(self, model, cfun, scheme, subjects, params=None,
                  kinds=None, ring_dtype='f32', ring_scale=1.0, waves=1,
                  steps_per_launch=1, ipl=1, min_waves=1, unroll=8, pipe=False,
-                 prio=True, queue=False, dt=None,
+                 prio=True, queue=False, emaj=0, dt=None,
                  context=None):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.subjects = [(csr, int(p)) for csr, p in subjects]
@@ -820,7 +832,7 @@ class Batch:
                                     sigma_kind=sigma_kind, ipl=ipl,
                                     ring_dtype=ring_dtype, unroll=unroll,
                                     min_waves=min_waves, pipe=pipe,
-                                    prio=prio)
+                                    prio=prio, emaj=emaj)
         if ring_dtype == 'u8':
             lo, hi = cfun.pre_bounds()
             if lo < 0 or hi * ring_scale > 1.0 + 1e-6:
