@@ -66,8 +66,12 @@ METRIC = ('instance·steps/s at N=998 sweep vs plain kernel; FC err vs '
 HBM_PEAK_GBS = 8000.0
 
 
+_T0 = time.perf_counter()
+
+
 def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+    print('[%6.1f s]' % (time.perf_counter() - _T0), *a, file=sys.stderr,
+          flush=True)
 
 
 def step_kernel(sim):
@@ -278,8 +282,8 @@ def cpu_baseline(plain_cfg, n_inst, n_steps, threads):
     idx = np.arange(n_inst)
     res = {}
     for variant in ('native', 'native_fastmath'):
-        lib = O.load_library(O.build_native(fast_math=variant.endswith(
-            'fastmath')))
+        lib = O.load_library(O.build_native(
+            fast_math=variant.endswith('fastmath')))
         ref = O.for_config(plain_cfg, idx, np.float32,
                            n_threads=threads, library=lib)
         ref.run(5, seed=seed)
@@ -686,6 +690,16 @@ def main():
         return bench_logp(args)
     legs = Legs()
     gw = 0 if args.no_graph else args.graph_window
+    if not args.no_cpu:
+        # the CPU baseline's -march=native builds compile on this host
+        # (one g++ each) while the GPU legs run
+        import threading
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        import oracle as O
+        for fm in (False, True):
+            threading.Thread(target=lambda fm=fm: legs(
+                'cpu_build', O.build_native, fast_math=fm),
+                daemon=True).start()
     cfg = configs.ALL[args.workload]()
     K = args.steps or cfg.n_steps
     W = args.warmup
@@ -715,7 +729,7 @@ def main():
                   args.profile_steps)
     if main_r is not None:
         main_r['roofline']['traffic'] = traffic_from_profile(
-            main_r['roofline']['kernel'], cfg.name, launch)
+            main_r['roofline']['kernel'], args.workload, launch)
         out['value'] = main_r['value']
         out['ms_per_step'] = main_r['ms_per_step']
         out['roofline'] = main_r['roofline']
@@ -728,7 +742,7 @@ def main():
                        args.profile_steps)
         if plain_r is not None:
             plain_r['roofline']['traffic'] = traffic_from_profile(
-                plain_r['roofline']['kernel'], pcfg.name, pl)
+                plain_r['roofline']['kernel'], args.plain, pl)
             out['plain'] = {'workload': '%s: %s' % (pcfg.name,
                                                     pcfg.description),
                             'value': plain_r['value'],

@@ -191,6 +191,13 @@ int tvbx_sim_set_exec(tvbx_sim *sim, int use_graph, int graph_window,
  * under 4 GiB; tvbx_sim_logp, tvbx_sim_checkpoint_run and tvbx_sim_adjoint
  * refuse mode 1.  The tables are rebuilt by tvbx_sim_set_connectome. */
 int tvbx_sim_set_coupling(tvbx_sim *sim, int mode);
+/* Pacing of blocked launches (no counterpart in the reference, which has
+ * no multi-step launches): on = 1 makes the workgroups that the hardware
+ * deals to one XCD (linear id mod 8) wait for each other after every step
+ * of a blocked launch, so that the rows of a step meet their readers in
+ * that XCD's L2.  Results do not depend on it; a wait is bounded and gives
+ * up pacing when it runs out. */
+int tvbx_sim_set_pacing(tvbx_sim *sim, int on);
 /* advance n_steps.  noise: optional injected normals (host), either
  * [n_steps][S][N][P] (tstride = S*N*P) or [S][N][P] reused each step
  * (tstride = 0). */

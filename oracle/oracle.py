@@ -14,6 +14,7 @@ instance-major and the wrapper transposes.
 import ctypes
 import os
 import subprocess
+import threading
 
 import numpy as np
 
@@ -156,12 +157,23 @@ def build_native(fast_math=False):
     checker (contraction and fast-math change the float32 rounding)."""
     sub = 'native_%s%s' % (_host_key(), '_fm' if fast_math else '')
     out = os.path.join(HERE, '_build', sub, 'libtvboracle.so')
-    if not os.path.exists(out):
-        subprocess.check_call(['make', '-s', '-C', HERE, 'native',
-                               'NATIVE_OUT=_build/' + sub,
-                               'NATIVE_FLAGS=' + ('-ffast-math'
-                                                  if fast_math else '')])
+    with _build_lock(sub):
+        if not os.path.exists(out):
+            subprocess.check_call(['make', '-s', '-C', HERE, 'native',
+                                   'NATIVE_OUT=_build/' + sub,
+                                   'NATIVE_FLAGS=' + ('-ffast-math'
+                                                      if fast_math else '')])
     return out
+
+
+_locks = {}
+_locks_guard = threading.Lock()
+
+
+def _build_lock(name):
+    "one build of each library at a time in this process"
+    with _locks_guard:
+        return _locks.setdefault(name, threading.Lock())
 
 
 def load_library(path=None):

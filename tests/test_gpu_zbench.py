@@ -5,6 +5,7 @@ import json
 import os
 import subprocess
 import sys
+import tempfile
 import time
 
 import pytest
@@ -15,17 +16,29 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 @pytest.mark.timeout(900)
-def test_bench_command_prints_a_complete_line():
+def test_bench_command_prints_a_complete_line(capsys):
     cmd = [sys.executable, 'bench.py', '--gpus', '1', '--steps', '20',
            '--warmup', '5']
     t0 = time.perf_counter()
-    r = subprocess.run(cmd, cwd=ROOT, stdout=subprocess.PIPE,
-                       stderr=subprocess.PIPE, text=True, timeout=850)
-    wall = time.perf_counter() - t0
-    print(r.stderr[-4000:])
-    assert r.returncode == 0, r.stderr[-4000:]
-    lines = [ln for ln in r.stdout.splitlines() if ln.startswith('{')]
-    assert len(lines) == 1, r.stdout
+    with tempfile.TemporaryFile(mode='w+') as fout:
+        proc = subprocess.Popen(cmd, cwd=ROOT, stdout=fout,
+                                stderr=subprocess.PIPE, text=True)
+        # the benchmark's progress lines, shown as they come (past pytest's
+        # output capture) while the command runs
+        err = []
+        for line in proc.stderr:
+            err.append(line)
+            with capsys.disabled():
+                sys.stdout.write('  bench: ' + line)
+                sys.stdout.flush()
+        rc = proc.wait(timeout=60)
+        wall = time.perf_counter() - t0
+        fout.seek(0)
+        out = fout.read()
+    err = ''.join(err)
+    assert rc == 0, err[-4000:]
+    lines = [ln for ln in out.splitlines() if ln.startswith('{')]
+    assert len(lines) == 1, out
     d = json.loads(lines[0])
     print(json.dumps(d, indent=1)[:6000])
     assert 'errors' not in d, d.get('errors')

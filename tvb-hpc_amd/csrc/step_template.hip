@@ -706,6 +706,27 @@ __device__ __forceinline__ void tvbx_coupling(
   }
 }
 
+// Wave priority from a launch's progress: 3, 2, 1, 0 over the four quarters
+// of its n steps.  The waves that share a SIMD in a multi-step launch with
+// no barriers start together; with equal priorities the oldest is issued
+// first, runs ahead and ends early, and the last waves finish the launch
+// alone, with nothing to hide their memory latency.  Falling priority issues
+// the lagging waves first, so the waves of a SIMD stay within a quarter of
+// the launch of each other and end together.
+__device__ __forceinline__ void tvbx_progress_prio(int k, int n) {
+#if TVBX_PRIO
+  const int q = (4 * k) / n;
+  if (q == 0)
+    __builtin_amdgcn_s_setprio(3);
+  else if (q == 1)
+    __builtin_amdgcn_s_setprio(2);
+  else if (q == 2)
+    __builtin_amdgcn_s_setprio(1);
+  else
+    __builtin_amdgcn_s_setprio(0);
+#endif
+}
+
 #if TVBX_EMAJ && !TVBX_ORDER_REF
 // Coupling sums of node i for all steps k < n of a blocked launch, edges
 // outer and steps inner.  Every read is of a slot written before the launch
@@ -760,6 +781,9 @@ __device__ __forceinline__ void tvbx_coupling_launch(
       // byte offsets of the edges' rows at the launch's first slot
       const unsigned ro = tvbx_edge_row(cdc, t, a) * rowb;
       for (int u0 = 0; u0 < nc; u0 += TVBX_UNROLL) {
+        // priority falls with the progress through the row, so that the
+        // waves walking the delays stay together
+        tvbx_progress_prio(cb + u0, deg);
         unsigned so[TVBX_UNROLL];
         float ws[TVBX_UNROLL];
 #pragma unroll
@@ -994,27 +1018,6 @@ __device__ __forceinline__ void tvbx_block(const tvbx_step_args &a, int &bx,
   }
 }
 
-// Wave priority from a launch's progress: 3, 2, 1, 0 over the four quarters
-// of its n steps.  The waves that share a SIMD in a multi-step launch with
-// no barriers start together; with equal priorities the oldest is issued
-// first, runs ahead and ends early, and the last waves finish the launch
-// alone, with nothing to hide their memory latency.  Falling priority issues
-// the lagging waves first, so the waves of a SIMD stay within a quarter of
-// the launch of each other and end together.
-__device__ __forceinline__ void tvbx_progress_prio(int k, int n) {
-#if TVBX_PRIO
-  const int q = (4 * k) / n;
-  if (q == 0)
-    __builtin_amdgcn_s_setprio(3);
-  else if (q == 1)
-    __builtin_amdgcn_s_setprio(2);
-  else if (q == 2)
-    __builtin_amdgcn_s_setprio(1);
-  else
-    __builtin_amdgcn_s_setprio(0);
-#endif
-}
-
 // step and ring slot of the launch's first step (graph replays add the
 // device clock)
 __device__ __forceinline__ void tvbx_start(const tvbx_step_args &a,
@@ -1137,10 +1140,57 @@ tvbx_step(const tvbx_step_args a) {
 // waves) and keeps its state and tavg accumulator in registers for all the
 // launch's steps.  At most 8 waves per workgroup (the state of 4 instances
 // per lane needs more than 128 registers).
+// Pacing of a blocked launch: after step k every workgroup of group
+// g = (linear workgroup id) mod 8 -- the hardware deals workgroup ids to the
+// 8 XCDs round robin, so a group is one XCD's share -- waits until all
+// workgroups of its group have finished step k, so that the rows a step
+// reads are read by all their target nodes of the XCD at about the same
+// time, while they are in that XCD's L2.  Correctness does not depend on it
+// (a blocked launch reads only slots written before it): a wait that spins
+// longer than its bound ends pacing for the workgroup (*on = 0).
+#define TVBX_PACE_SPINS 20000
+__device__ __forceinline__ void tvbx_pace(const tvbx_step_args &a, int k,
+                                          int *on) {
+  __syncthreads();
+  if (threadIdx.x == 0 && *on) {
+    const unsigned G = gridDim.x * gridDim.y;
+    const unsigned w = blockIdx.x + gridDim.x * blockIdx.y;
+    const unsigned g = w & 7u;
+    const unsigned cnt = (G - g + 7u) / 8u;
+    unsigned *c = a.pace + 16 * g;
+    atomicAdd(c, 1u);
+    const unsigned want = (unsigned)(k + 1) * cnt;
+    int spins = 0;
+    while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+           want) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > TVBX_PACE_SPINS) {
+        *on = 0;
+        break;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// the last workgroup of a paced launch to finish clears the counters for
+// the next launch on the stream
+__device__ __forceinline__ void tvbx_pace_end(const tvbx_step_args &a) {
+  if (threadIdx.x == 0) {
+    const unsigned G = gridDim.x * gridDim.y;
+    if (atomicAdd(a.pace + 128, 1u) == G - 1u) {
+#pragma unroll
+      for (int g = 0; g < 8; ++g) atomicExch(a.pace + 16 * g, 0u);
+      atomicExch(a.pace + 128, 0u);
+    }
+  }
+}
+
 // The blocked launch of one (instance tile bx, node i) pair: n_steps steps
 // of that node with its state in registers.
 __device__ __forceinline__ void tvbx_blocked_run(const tvbx_step_args &a,
-                                                 int bx, int i) {
+                                                 int bx, int i,
+                                                 int *pace_on = nullptr) {
   const int lane = threadIdx.x & 63;
   const int pb = (bx * 64 + lane) * TVBX_IPL;
   const int N = a.n_node;
@@ -1217,6 +1267,7 @@ __device__ __forceinline__ void tvbx_blocked_run(const tvbx_step_args &a,
       tvbx_str(ring_w + tvbx_ring_off(t1, r, i, a) + pb, yo[r]);
     if (a.tavg_period > 0) tvbx_monitor(a, ip, n, y_mon, acc_t);
     t = t1;
+    if (pace_on != nullptr && k + 1 < a.n_steps) tvbx_pace(a, k, pace_on);
   }
   // An unforced update keeps a non-finite state non-finite (x itself is a
   // term of x + dt f + sqrt(dt) g xi, and the wrap subtracts a constant),
@@ -1243,8 +1294,22 @@ tvbx_step_blocked(const tvbx_step_args a) {
   tvbx_block(a, bx, by);
   const int i = __builtin_amdgcn_readfirstlane(by * a.node_block +
                                                (int)(threadIdx.x >> 6));
-  if (i >= a.n_node) return;
-  tvbx_blocked_run(a, bx, i);
+  if (a.pace == nullptr) {
+    if (i >= a.n_node) return;
+    tvbx_blocked_run(a, bx, i);
+    return;
+  }
+  // paced: every wave of the workgroup takes part in the waits, also one
+  // past the last node
+  __shared__ int pace_on;
+  if (threadIdx.x == 0) pace_on = 1;
+  __syncthreads();
+  if (i < a.n_node) {
+    tvbx_blocked_run(a, bx, i, &pace_on);
+  } else {
+    for (int k = 0; k + 1 < a.n_steps; ++k) tvbx_pace(a, k, &pace_on);
+  }
+  tvbx_pace_end(a);
 }
 
 // Batched sweep of subjects with different connectomes (sizes, delays) in

@@ -127,6 +127,13 @@ typedef struct tvbx_step_args {
   const unsigned short *mfma_w;
   float mfma_unscale;
   float *csum;
+  /* pacing of blocked launches (null: off): workgroups in groups of linear
+   * id mod 8 (the XCD the hardware deals them to) wait for each other after
+   * every step, counting in pace[16 g]; pace[128] counts finished
+   * workgroups, and the last one clears the counters.  The waits only pace
+   * the launch (no data passes through them) and give up after a bounded
+   * spin. */
+  unsigned *pace;
 } tvbx_step_args;
 
 /* Argument block of the batched step kernel: per subject (one connectome

@@ -212,6 +212,9 @@ struct tvbx_sim {
   bool ck_recorded = false;
   DevBuf lam, mu, rec_x, rec_sum, forcing, g_c, g_m, g_out, save_state,
       save_ring;
+  // pacing of blocked launches (tvbx_sim_set_pacing)
+  bool pace = false;
+  DevBuf pace_buf;
   // delay-binned coupling on the matrix cores (tvbx_sim_set_coupling)
   bool mfma = false;
   DevBuf mfma_blk, mfma_pairs, mfma_w, csum;
@@ -613,16 +616,23 @@ __global__ void k_fc_mean(int N, int pitch, int count, double n_prev,
 }
 
 // corr_ij = M2_ij / sqrt(M2_ii M2_jj) (metric.py:111-139; the 1/n of the
-// covariance cancels)
-__global__ void k_fc_corr(const double *__restrict__ m2, int N, int pitch,
-                          float *__restrict__ corr) {
-  const int p = blockIdx.x * 64 + threadIdx.x;
+// covariance cancels) for instances p0 .. p0 + np - 1, both triangles, into
+// corr [np][N][N].  grid (ceil(np / 64), N), block 64.
+__global__ void k_fc_corr_full(const double *__restrict__ m2, int N,
+                               int pitch, int p0, int np,
+                               float *__restrict__ corr) {
+  const int q = blockIdx.x * 64 + threadIdx.x;
   const int i = blockIdx.y;
+  if (q >= np) return;
+  const int p = p0 + q;
   const double mii = m2[tri_index(i, i, N) * pitch + p];
+  float *cq = corr + (size_t)q * N * N;
   for (int j = i; j < N; ++j) {
     const double mjj = m2[tri_index(j, j, N) * pitch + p];
-    const size_t idx = tri_index(i, j, N) * pitch + p;
-    corr[idx] = (float)(m2[idx] / sqrt(mii * mjj));
+    const float v =
+        (float)(m2[tri_index(i, j, N) * pitch + p] / sqrt(mii * mjj));
+    cq[(size_t)i * N + j] = v;
+    cq[(size_t)j * N + i] = v;
   }
 }
 
@@ -811,6 +821,7 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
   a.ring_unscale = d.ring_type == 0   ? 1.0f
                    : d.ring_type == 3 ? 1.0f / (255.0f * d.ring_scale)
                                       : 1.0f / d.ring_scale;
+  if (s->pace) a.pace = (unsigned *)s->pace_buf.p;
   if (s->mfma) {
     a.mfma_blk = (const int *)s->mfma_blk.p;
     a.mfma_pairs = (const unsigned *)s->mfma_pairs.p;
@@ -985,7 +996,8 @@ extern "C" int tvbx_sim_destroy(tvbx_sim *s) {
                    &s->rec_sum, &s->forcing, &s->g_c, &s->g_m, &s->g_out,
                    &s->save_state, &s->save_ring};
   for (DevBuf *b : adj) dfree(*b);
-  DevBuf *mf[] = {&s->mfma_blk, &s->mfma_pairs, &s->mfma_w, &s->csum};
+  DevBuf *mf[] = {&s->mfma_blk, &s->mfma_pairs, &s->mfma_w, &s->csum,
+                  &s->pace_buf};
   for (DevBuf *b : mf) dfree(*b);
   for (auto &b : s->ck_state) dfree(b);
   for (auto &b : s->ck_ring) dfree(b);
@@ -1165,6 +1177,21 @@ static int mfma_tables(tvbx_sim *s) {
   }
   s->mfma_unscale = std::ldexp(1.0f, ex);
   s->mfma_chunks = blk[nb];
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_sim_set_pacing(tvbx_sim *s, int on) {
+  CHECK(s && (on == 0 || on == 1), "bad argument");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  if (on && !s->pace_buf.p) {
+    int rc = dalloc(s->pace_buf, 256 * sizeof(unsigned));
+    if (rc != TVBX_OK) return rc;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s->ctx->stream,
+                       (unsigned *)s->pace_buf.p, (size_t)256, 0u);
+    HIPCHK(hipGetLastError());
+  }
+  if ((on != 0) != s->pace) drop_graphs(s);
+  s->pace = on != 0;
   return TVBX_OK;
 }
 
@@ -1693,28 +1720,19 @@ extern "C" int tvbx_sim_fc(tvbx_sim *s, int p0, int np, float *corr,
   if (!corr || np == 0) return TVBX_OK;
   CHECK(s->samples_used >= 2, "FC needs at least two samples");
   const int N = d.n_node;
-  const size_t tri = (size_t)N * (N + 1) / 2;
-  rc = dalloc(s->corr, tri * s->pitch * 4);
+  // the requested instances' matrices, both triangles, in one buffer: one
+  // linear copy (a strided copy of the triangle's N (N + 1) / 2 rows per
+  // call took seconds at N = 998)
+  const size_t nn = (size_t)N * N;
+  rc = dalloc(s->corr, (size_t)np * nn * 4);
   if (rc != TVBX_OK) return rc;
-  hipLaunchKernelGGL(k_fc_corr, dim3(s->pitch / 64, N), dim3(64), 0,
+  hipLaunchKernelGGL(k_fc_corr_full, dim3((np + 63) / 64, N), dim3(64), 0,
                      s->ctx->stream, (const double *)s->fc_m2.p, N, s->pitch,
-                     (float *)s->corr.p);
+                     p0, np, (float *)s->corr.p);
   HIPCHK(hipGetLastError());
-  std::vector<float> host(tri * (size_t)np);
-  HIPCHK(hipMemcpy2DAsync(host.data(), (size_t)np * 4,
-                          (const float *)s->corr.p + p0, (size_t)s->pitch * 4,
-                          (size_t)np * 4, tri, hipMemcpyDeviceToHost,
-                          s->ctx->stream));
+  HIPCHK(hipMemcpyAsync(corr, s->corr.p, (size_t)np * nn * 4,
+                        hipMemcpyDeviceToHost, s->ctx->stream));
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
-  for (int q = 0; q < np; ++q) {
-    float *cq = corr + (size_t)q * N * N;
-    for (int i = 0; i < N; ++i)
-      for (int j = i; j < N; ++j) {
-        float v = host[tri_index(i, j, N) * np + q];
-        cq[(size_t)i * N + j] = v;
-        cq[(size_t)j * N + i] = v;
-      }
-  }
   return TVBX_OK;
 }
 

@@ -102,6 +102,7 @@ _SIGS = {
     'tvbx_sim_run': [_P, _LL, _P, _LL],
     'tvbx_sim_set_exec': [_P, _I, _I, _I],
     'tvbx_sim_set_coupling': [_P, _I],
+    'tvbx_sim_set_pacing': [_P, _I],
     'tvbx_sim_checkpoint_run': [_P, _LL, _I],
     'tvbx_sim_adjoint': [_P, _P, _I, _LL, _LL, _I, _P, _I, _P, _I],
     'tvbx_sim_get_outputs': [_P, _P, _P, _P],
@@ -380,7 +381,7 @@ class Simulation:
         their model / coupling defaults as literals.
     monitor: tavg_period (steps), bold (bool), fc (bool), sample_slots.
     launch: ipl, waves, node_block, steps_per_launch, unroll, min_waves,
-        pipe, prio, xcd_map, emaj (see tvb_hpc.autotune); coupling 'gather' (CSR
+        pipe, prio, xcd_map, emaj, pace (see tvb_hpc.autotune); coupling 'gather' (CSR
         rows) or 'mfma' (delay-binned sums on the matrix cores; bfloat16 /
         float16 rings holding pre(y), blocked launches of at most 16 steps).
     """
@@ -393,7 +394,7 @@ class Simulation:
                  order='cuda', ipl=1, ring_dtype='f32', ring_scale=1.0,
                  adjoint=False, unroll=8, min_waves=1, pipe=False,
                  prio=True, xcd_map=False, coupling='gather', emaj=0,
-                 context=None):
+                 pace=False, context=None):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.csr = csr
         N, P = csr.nnode, int(n_inst)
@@ -497,6 +498,8 @@ class Simulation:
         self.set_params(resolved)
         self.coupling = None
         self.set_coupling(coupling)
+        if pace:
+            check(lib().tvbx_sim_set_pacing(self.h, 1))
 
     def set_coupling(self, mode):
         """'gather': CSR row sums; 'mfma': delay-binned sums on the matrix
