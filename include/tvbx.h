@@ -299,6 +299,20 @@ int tvbx_rng_normals(tvbx_ctx *ctx, unsigned long long seed, float *out,
  * [n][N][P] (host) -> corr [P][N][N] (host), FP64 accumulation. */
 int tvbx_corr(tvbx_ctx *ctx, const float *samples, long long n, int N, int P,
               float *corr);
+/* Allocation-free variant of tvbx_corr, with no counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * samples [n][N][P] and corr [P][N][N] are device pointers; every
+ * intermediate (FP64 co-moments and means, block sums, the tile table, a
+ * pitched copy of the samples when P is not a multiple of 64) lives in one
+ * fixed 128 MiB device scratch array per GPU, whose address
+ * tvbx_ctx_create resolves once per device.  Sizes that need more scratch
+ * return TVBX_E_ARG with the sizes in the message.  Nothing is allocated,
+ * freed or copied to or from the host; the work is queued on the context's
+ * stream (tvbx_ctx_sync waits for it).  Calls that overlap on one GPU (from
+ * several contexts or threads) share the scratch and race. */
+int tvbx_corr_noalloc(tvbx_ctx *ctx, const float *samples, long long n,
+                      int N, int P, float *corr);
 
 /* ---- region mapping: surface vertices <-> regions ---------------------------
  * Reference: RMapToAvg / RMapFromAvg (tvb_hpc/network.py:174-202; known

@@ -20,6 +20,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -258,6 +259,34 @@ extern "C" int tvbx_device_count(int *n) {
   return TVBX_OK;
 }
 
+// Device scratch of the allocation-free entry point (tvbx_corr_noalloc): a
+// file-scope array that every GPU the library runs on holds from the load of
+// the library's code object until the process ends.  On the host the name
+// only stands in for the symbol, so no kernel argument ever uses it: each
+// device's own copy is resolved once, with that device current, by
+// hipGetSymbolAddress (tvbx_ctx_create), and the kernels take that address.
+#define TVBX_SCRATCH_BYTES ((size_t)128 << 20)
+#define TVBX_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[TVBX_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[TVBX_MAX_DEVICES];
+
+// the scratch address on `device`, which is the current device
+static int scratch_resolve(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  CHECK(device >= 0 && device < TVBX_MAX_DEVICES,
+        "device %d beyond the %d devices the scratch table holds", device,
+        TVBX_MAX_DEVICES);
+  if (g_scratch_ptr[device]) return TVBX_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr)
+    return fail(TVBX_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s",
+                device, hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return TVBX_OK;
+}
+
 extern "C" int tvbx_ctx_create(int device, tvbx_ctx **out) {
   CHECK(out, "null argument");
   int n = 0;
@@ -273,6 +302,12 @@ extern "C" int tvbx_ctx_create(int device, tvbx_ctx **out) {
     return fail(TVBX_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
   }
   HIPCHK(hipGetDeviceProperties(&c->prop, device));
+  const int rc = scratch_resolve(device);
+  if (rc != TVBX_OK) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
   *out = c;
   return TVBX_OK;
 }
@@ -2020,6 +2055,105 @@ extern "C" int tvbx_corr(tvbx_ctx *c, const float *samples, long long n,
                    &s.fc_s1, &s.fc_u};
   for (DevBuf *b : all) dfree(*b);
   return rc;
+}
+
+// ------------------------------------------------- allocation-free FC
+// The tile table of k_fc_update in super-tile order (fc_tile_order), built
+// on the device by one thread.
+__global__ void k_fc_tile_table(int T, int2 *__restrict__ t) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int k = 0;
+  const int TS = (T + FC_ST - 1) / FC_ST;
+  for (int si = 0; si < TS; ++si)
+    for (int sj = si; sj < TS; ++sj)
+      for (int I = si * FC_ST; I < min(T, si * FC_ST + FC_ST); ++I)
+        for (int J = max(I, sj * FC_ST); J < min(T, sj * FC_ST + FC_ST); ++J)
+          t[k++] = make_int2(I, J);
+}
+
+// rows [n * N][P] -> [n * N][pitch], the padding lanes zero
+__global__ void k_pitch_rows(const float *__restrict__ in, size_t rows, int P,
+                             int pitch, float *__restrict__ out) {
+  const size_t total = rows * (size_t)pitch;
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; k < total; k += stride) {
+    const size_t r = k / pitch;
+    const int p = (int)(k - r * pitch);
+    out[k] = p < P ? in[r * P + p] : 0.0f;
+  }
+}
+
+// tvbx_corr on device memory with every intermediate in the device's
+// scratch array: no allocation, no host copy, no synchronisation.
+extern "C" int tvbx_corr_noalloc(tvbx_ctx *c, const float *samples,
+                                 long long n, int N, int P, float *corr) {
+  CHECK(c && samples && corr, "null argument");
+  CHECK(n >= 2 && N >= 1 && P >= 1, "need n >= 2 samples, N, P >= 1");
+  CHECK(n <= (1ll << 30), "too many samples");
+  CHECK(c->device >= 0 && c->device < TVBX_MAX_DEVICES &&
+            g_scratch_ptr[c->device] != nullptr,
+        "no scratch address for device %d", c->device);
+  const int pitch = (P + 63) / 64 * 64;
+  const size_t NP = (size_t)N * pitch;
+  const size_t tri = (size_t)N * (N + 1) / 2;
+  const int T = (N + FC_TILE - 1) / FC_TILE;
+  const int n_tiles = T * (T + 1) / 2;
+  const int n_fblk = pitch / FC_PB;
+  const bool stage = pitch != P;  // the FC kernels read [n][N][pitch]
+  // the scratch's pieces, each 256-byte aligned
+  size_t used = 0;
+  auto take = [&used](size_t bytes) {
+    const size_t o = used;
+    used += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t o_m2 = take(tri * pitch * 8), o_mean = take(NP * 8),
+               o_s1 = take(NP * 8), o_u = take(NP * 8),
+               o_shift = take(NP * 4), o_tiles = take((size_t)n_tiles * 8),
+               o_smp = stage ? take((size_t)n * NP * 4) : 0;
+  CHECK(used <= TVBX_SCRATCH_BYTES,
+        "tvbx_corr_noalloc: n=%lld, N=%d, P=%d need %zu bytes of device "
+        "scratch; the limit is %zu bytes",
+        n, N, P, used, (size_t)TVBX_SCRATCH_BYTES);
+  const unsigned long long nwg =
+      ((unsigned long long)n_tiles * n_fblk + 7) / 8 * 8;
+  CHECK(nwg < (1ull << 31), "FC problem too large for one launch");
+  HIPCHK(hipSetDevice(c->device));
+  char *base = static_cast<char *>(g_scratch_ptr[c->device]);
+  double *m2 = reinterpret_cast<double *>(base + o_m2);
+  double *mean = reinterpret_cast<double *>(base + o_mean);
+  double *s1 = reinterpret_cast<double *>(base + o_s1);
+  double *u = reinterpret_cast<double *>(base + o_u);
+  float *shift = reinterpret_cast<float *>(base + o_shift);
+  int2 *tiles = reinterpret_cast<int2 *>(base + o_tiles);
+  const float *smp = samples;
+  hipStream_t st = c->stream;
+  // the scratch keeps the previous call's contents: the co-moments and
+  // means start from zero (fill kernels, so the call can be captured)
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid1d(tri * pitch, 256)), dim3(256),
+                     0, st, m2, tri * pitch, 0.0);
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid1d(NP, 256)), dim3(256), 0, st,
+                     mean, NP, 0.0);
+  hipLaunchKernelGGL(k_fc_tile_table, dim3(1), dim3(64), 0, st, T, tiles);
+  if (stage) {
+    float *sp = reinterpret_cast<float *>(base + o_smp);
+    hipLaunchKernelGGL(k_pitch_rows, dim3(grid1d((size_t)n * NP, 256)),
+                       dim3(256), 0, st, samples, (size_t)n * N, P, pitch,
+                       sp);
+    smp = sp;
+  }
+  hipLaunchKernelGGL(k_fc_prep, dim3(pitch / 64, N), dim3(64), 0, st, smp,
+                     (int)n, 0ll, (int)n, N, pitch, 0.0, mean, shift, s1, u);
+  hipLaunchKernelGGL(k_fc_update, dim3((unsigned)nwg), dim3(64 * FC_NW), 0,
+                     st, smp, (int)n, 0ll, (int)n, N, pitch, 0.0, tiles,
+                     n_tiles, n_fblk, shift, s1, u, m2);
+  hipLaunchKernelGGL(k_fc_mean, dim3(pitch / 64, N), dim3(64), 0, st, N,
+                     pitch, (int)n, 0.0, u, mean);
+  hipLaunchKernelGGL(k_fc_corr_full, dim3((P + 63) / 64, N), dim3(64), 0, st,
+                     m2, N, pitch, 0, P, corr);
+  HIPCHK(hipGetLastError());
+  return TVBX_OK;
 }
 
 // ------------------------------------------------------------ region maps

@@ -115,6 +115,7 @@ _SIGS = {
     'tvbx_sim_timing': [_P, _P, _P, _I],
     'tvbx_rng_normals': [_P, ctypes.c_ulonglong, _P, _SZ],
     'tvbx_corr': [_P, _P, _LL, _I, _I, _P],
+    'tvbx_corr_noalloc': [_P, _P, _LL, _I, _I, _P],
     'tvbx_rmap_create': [_P, _P, _I, _I, _P],
     'tvbx_rmap_destroy': [_P],
     'tvbx_rmap_to_avg': [_P, _I, _I, _P, _P],
@@ -267,6 +268,15 @@ class Context:
         out = np.zeros((P, N, N), np.float32)
         check(lib().tvbx_corr(self.h, _ptr(s), n, N, P, _ptr(out)))
         return out
+
+    def corr_noalloc(self, samples_ptr, n, N, P, corr_ptr):
+        """tvbx_corr on device memory with no allocation: samples
+        (n, N, P) and corr (P, N, N) float32 at the device addresses given
+        (e.g. torch tensors' data_ptr()); queued on this context's stream
+        (``sync()`` waits for it)."""
+        check(lib().tvbx_corr_noalloc(self.h, ctypes.c_void_p(samples_ptr),
+                                      int(n), int(N), int(P),
+                                      ctypes.c_void_p(corr_ptr)))
 
 
 class RegionMap:
