@@ -192,11 +192,12 @@ int tvbx_sim_set_exec(tvbx_sim *sim, int use_graph, int graph_window,
  * refuse mode 1.  The tables are rebuilt by tvbx_sim_set_connectome. */
 int tvbx_sim_set_coupling(tvbx_sim *sim, int mode);
 /* Pacing of blocked launches (no counterpart in the reference, which has
- * no multi-step launches): on = 1 makes the workgroups that the hardware
- * deals to one XCD (linear id mod 8) wait for each other after every step
- * of a blocked launch, so that the rows of a step meet their readers in
- * that XCD's L2.  Results do not depend on it; a wait is bounded and gives
- * up pacing when it runs out. */
+ * no multi-step launches): on = lag >= 1 makes each workgroup of the ones
+ * the hardware deals to one XCD (linear id mod 8), after step k of a
+ * blocked launch, wait until all of them have finished step k + 1 - lag
+ * (1: lockstep), so that the rows of a step meet their readers in that
+ * XCD's L2; 0 switches it off.  Results do not depend on it; a wait is
+ * bounded and gives up pacing when it runs out. */
 int tvbx_sim_set_pacing(tvbx_sim *sim, int on);
 /* advance n_steps.  noise: optional injected normals (host), either
  * [n_steps][S][N][P] (tstride = S*N*P) or [S][N][P] reused each step

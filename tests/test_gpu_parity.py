@@ -738,3 +738,23 @@ def test_edge_major_long_rows_equal_step_major():
         sim.run(4 * 25 + 2)
         out.append(sim.get_state())
     np.testing.assert_array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize('ipl,xcd', [(1, False), (2, True)])
+def test_pacing_changes_nothing(ipl, xcd):
+    """Workgroups of one XCD waiting for each other after every step of a
+    blocked launch only change the timing: the state equals the unpaced
+    run's bit for bit, over several launches (the counters must come back
+    to zero after each) and a shorter last launch."""
+    out = []
+    for pace in (False, True):
+        sim, _, _ = _pair('JansenRit', 'Sigmoidal', 0.05, {'a': 0.3}, P=256,
+                          steps=1, em=False, ring_dtype='bf16',
+                          ring_scale=200.0,
+                          launch=dict(ipl=ipl, waves=2, node_block=2,
+                                      steps_per_launch=8, xcd_map=xcd,
+                                      pace=pace))
+        sim.run(8 * 12 + 5)
+        out.append(sim.get_state())
+        assert not sim.flags().any()
+    np.testing.assert_array_equal(out[0], out[1])

@@ -1143,7 +1143,8 @@ tvbx_step(const tvbx_step_args a) {
 // Pacing of a blocked launch: after step k every workgroup of group
 // g = (linear workgroup id) mod 8 -- the hardware deals workgroup ids to the
 // 8 XCDs round robin, so a group is one XCD's share -- waits until all
-// workgroups of its group have finished step k, so that the rows a step
+// workgroups of its group have finished step k + 1 - pace_lag (lag 1:
+// lockstep; lag 2: one step of slack), so that the rows a step
 // reads are read by all their target nodes of the XCD at about the same
 // time, while they are in that XCD's L2.  Correctness does not depend on it
 // (a blocked launch reads only slots written before it): a wait that spins
@@ -1159,7 +1160,9 @@ __device__ __forceinline__ void tvbx_pace(const tvbx_step_args &a, int k,
     const unsigned cnt = (G - g + 7u) / 8u;
     unsigned *c = a.pace + 16 * g;
     atomicAdd(c, 1u);
-    const unsigned want = (unsigned)(k + 1) * cnt;
+    // finished step k: wait for the group to finish step k + 1 - lag
+    const int done = k + 2 - a.pace_lag;
+    const unsigned want = done > 0 ? (unsigned)done * cnt : 0u;
     int spins = 0;
     while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
            want) {

@@ -132,8 +132,10 @@ typedef struct tvbx_step_args {
    * every step, counting in pace[16 g]; pace[128] counts finished
    * workgroups, and the last one clears the counters.  The waits only pace
    * the launch (no data passes through them) and give up after a bounded
-   * spin. */
+   * spin.  pace_lag: a workgroup that has finished step k waits for its
+   * group to finish step k + 1 - pace_lag (1: lockstep). */
   unsigned *pace;
+  int pace_lag;
 } tvbx_step_args;
 
 /* Argument block of the batched step kernel: per subject (one connectome

@@ -213,8 +213,8 @@ struct tvbx_sim {
   bool ck_recorded = false;
   DevBuf lam, mu, rec_x, rec_sum, forcing, g_c, g_m, g_out, save_state,
       save_ring;
-  // pacing of blocked launches (tvbx_sim_set_pacing)
-  bool pace = false;
+  // pacing of blocked launches (tvbx_sim_set_pacing): 0 off, else the lag
+  int pace = 0;
   DevBuf pace_buf;
   // delay-binned coupling on the matrix cores (tvbx_sim_set_coupling)
   bool mfma = false;
@@ -856,7 +856,10 @@ static void fill_args(tvbx_sim *s, tvbx_step_args &a) {
   a.ring_unscale = d.ring_type == 0   ? 1.0f
                    : d.ring_type == 3 ? 1.0f / (255.0f * d.ring_scale)
                                       : 1.0f / d.ring_scale;
-  if (s->pace) a.pace = (unsigned *)s->pace_buf.p;
+  if (s->pace) {
+    a.pace = (unsigned *)s->pace_buf.p;
+    a.pace_lag = s->pace;
+  }
   if (s->mfma) {
     a.mfma_blk = (const int *)s->mfma_blk.p;
     a.mfma_pairs = (const unsigned *)s->mfma_pairs.p;
@@ -1216,7 +1219,7 @@ static int mfma_tables(tvbx_sim *s) {
 }
 
 extern "C" int tvbx_sim_set_pacing(tvbx_sim *s, int on) {
-  CHECK(s && (on == 0 || on == 1), "bad argument");
+  CHECK(s && on >= 0 && on <= 8, "bad argument");
   HIPCHK(hipSetDevice(s->ctx->device));
   if (on && !s->pace_buf.p) {
     int rc = dalloc(s->pace_buf, 256 * sizeof(unsigned));
@@ -1225,8 +1228,8 @@ extern "C" int tvbx_sim_set_pacing(tvbx_sim *s, int on) {
                        (unsigned *)s->pace_buf.p, (size_t)256, 0u);
     HIPCHK(hipGetLastError());
   }
-  if ((on != 0) != s->pace) drop_graphs(s);
-  s->pace = on != 0;
+  if (on != s->pace) drop_graphs(s);
+  s->pace = on;
   return TVBX_OK;
 }
 

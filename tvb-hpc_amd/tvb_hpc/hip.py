@@ -509,7 +509,7 @@ class Simulation:
         self.coupling = None
         self.set_coupling(coupling)
         if pace:
-            check(lib().tvbx_sim_set_pacing(self.h, 1))
+            check(lib().tvbx_sim_set_pacing(self.h, int(pace)))
 
     def set_coupling(self, mode):
         """'gather': CSR row sums; 'mfma': delay-binned sums on the matrix
