@@ -93,6 +93,31 @@ def traffic_from_profile(kernel, workload, launch):
     return float(m.group(1))
 
 
+def device_copy_rate(mib=256, reps=20):
+    """HBM rate of a plain streaming copy on this device (torch's
+    elementwise kernel, `mib` MiB read and `mib` MiB written per call), as
+    the practical ceiling the step kernels' measured traffic compares with.
+    Timed with events on torch's current stream, where the copies run."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, device='cuda:0')
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch.mul(x, 1.0, out=y)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del x, y
+    return {'GBs': 2 * n * 4 / (ms * 1e-3) / 1e9, 'us_per_copy': ms * 1e3,
+            'what': 'torch.mul(x, 1.0, out=y) over %d MiB (read + write), '
+                    'mean of %d calls' % (mib, reps)}
+
+
 def distinct_sources(csr):
     """Distinct (source, delay) pairs per delay group: the history values a
     step must read at least once per instance (the compulsory gathers)."""
@@ -449,7 +474,7 @@ def bench_logp(args):
         'n_gpus': 1, 'steps': T, 'warmup': args.warmup,
         'ms_per_step': res['logp'] / T * 1e3, 'higher_is_better': True,
         'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'f32 state, f64 sums', 'data': 'synthetic',
+        'dtype': 'f32 state, compensated f32 sums', 'data': 'synthetic',
         'config': {'workload': '%s: EM log-likelihood of a %d-transition '
                                'trajectory of %s' % (args.workload, T,
                                                      cfg.description),
@@ -728,11 +753,21 @@ def main():
     main_r = legs('main', time_sweep, cfg, launch, K, W, gw,
                   args.profile_steps)
     if main_r is not None:
-        main_r['roofline']['traffic'] = traffic_from_profile(
-            main_r['roofline']['kernel'], args.workload, launch)
+        rf = main_r['roofline']
+        rf['traffic'] = traffic_from_profile(rf['kernel'], args.workload,
+                                             launch)
+        cp = legs('copy', device_copy_rate)
+        if cp is not None:
+            if rf['traffic']:
+                # measured HBM bytes per launch over the measured launch time
+                cp['step_kernel_traffic_GBs'] = \
+                    rf['traffic'] / (rf['launch_ms'] * 1e-3) / 1e9
+                cp['step_kernel_vs_copy'] = \
+                    cp['step_kernel_traffic_GBs'] / cp['GBs']
+            rf['device_copy'] = cp
         out['value'] = main_r['value']
         out['ms_per_step'] = main_r['ms_per_step']
-        out['roofline'] = main_r['roofline']
+        out['roofline'] = rf
     plain_r, pcfg, pl = None, None, None
     if args.plain != 'none' and args.plain != args.workload:
         pcfg = configs.ALL[args.plain]()
