@@ -213,8 +213,8 @@ def time_gradient(n_evals=2, window=1000):
     """Gradient evaluations/s of the FC-fit workload, end to end on the
     device: checkpointed forward of all instances (a checkpoint every
     `window` steps, replayed in the backward pass; window = 0 records every
-    step instead), FC loss against a target FC (from a run at G = 0.5) in
-    torch, and the adjoint."""
+    step instead), FC loss against a target FC (from a run at G = 0.5) on
+    the device (autograd.fc_loss_device), and the adjoint."""
     import numpy as np
     import torch
     from tvb_hpc import autograd as AG
@@ -241,7 +241,7 @@ def time_gradient(n_evals=2, window=1000):
         t = time.perf_counter()
         s = AG.integrate(sim, {'c:a': Gt, 'm:eta': et}, x0, T, window,
                          seed=5)
-        loss = AG.fc_loss(s, fct).sum()
+        loss = AG.fc_loss_device(s, fct).sum()
         loss.backward()
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t)
@@ -257,9 +257,10 @@ def time_gradient(n_evals=2, window=1000):
             'evals_per_s': 1e3 / ms, 'ms_per_eval': ms,
             'instance_gradients_per_s': P * 1e3 / ms,
             'params': ['c:a (P)', 'm:eta (N x P)'], 'launch': la,
-            'note': 'one evaluation = forward (recording x(n) and the '
-                    'coupling sums), FC loss in torch on the device, reverse '
-                    'sweep, for all instances'}
+            'note': 'one evaluation = checkpointed forward, FC loss on the '
+                    'device (FP64 co-moments, float32 gradient), replay of '
+                    'each window recording x(n) and the coupling sums, '
+                    'reverse sweep, for all instances'}
 
 
 def host_info():

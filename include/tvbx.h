@@ -315,6 +315,31 @@ int tvbx_corr(tvbx_ctx *ctx, const float *samples, long long n, int N, int P,
 int tvbx_corr_noalloc(tvbx_ctx *ctx, const float *samples, long long n,
                       int N, int P, float *corr);
 
+/* ---- FC-fit loss ------------------------------------------------------------
+ * No counterpart in the reference, which has no loss and no gradient (its
+ * README.md:3-4 names Bayesian inversion as the use): the FC-fit loss of the
+ * inversion workload and its derivative, for tvb_hpc.autograd.fc_loss_device.
+ * samples [n][N][P] and target [N][N] (shared) or [N][N][P]
+ * (target_per_instance) are device arrays; the outputs, also on the device:
+ *   loss [P]    1 - the Pearson correlation of the upper triangles (i < j) of
+ *               the instance's FC (FP64 co-moments, as tvbx_corr) and of the
+ *               target;
+ *   s [N][N][P] the symmetric matrix with
+ *               dloss[p]/dsamples[m][k][p] = sum_j s[k][j][p] *
+ *               (samples[m][j][p] - mean[j][p]);
+ *   mean [N][P] the samples' means.
+ * Queued on the context's stream; the context keeps a device workspace for
+ * the FP64 co-moments (grown on demand, freed by tvbx_ctx_destroy). */
+int tvbx_fc_loss(tvbx_ctx *ctx, const float *samples, long long n, int N,
+                 int P, const float *target, int target_per_instance,
+                 float *loss, float *s, float *mean);
+/* out[c][r] = in[r][c] - sub[r % sub_rows][c] (sub may be null): device
+ * arrays, in [rows][cols] -> out [cols][rows].  No counterpart in the
+ * reference: it moves the FC-loss gradient's operands between the
+ * instance-innermost layout and the instance-major one of batched products. */
+int tvbx_transpose(tvbx_ctx *ctx, const float *in, long long rows,
+                   long long cols, const float *sub, int sub_rows, float *out);
+
 /* ---- region mapping: surface vertices <-> regions ---------------------------
  * Reference: RMapToAvg / RMapFromAvg (tvb_hpc/network.py:174-202; known
  * answers tvb_hpc/tests.py:453-473).  rmap[k] in [0, nroi) is the region of

@@ -209,6 +209,30 @@ def test_autograd_device_tensors_match_host_tensors():
                                    atol=1e-6 * np.abs(a).max())
 
 
+def test_device_fc_loss_gradients_match_torch_loss():
+    """The parameter gradients through the device FC loss
+    (autograd.fc_loss_device) equal those through the float64 torch loss
+    (autograd.fc_loss) on the same device samples."""
+    csr, m, c, sc, G, eta, x0 = _setup()
+    s_t = _oracle_samples(csr, np.full(P, 0.5), eta, x0)
+    fct = AG.fc(torch.from_numpy(s_t))[0].numpy()
+    out = []
+    for loss_fn in (AG.fc_loss, AG.fc_loss_device):
+        sim = _device_sim(csr, m, c, sc, G, eta)
+        Gt = torch.tensor(G, device='cuda', requires_grad=True)
+        et = torch.tensor(eta, device='cuda', requires_grad=True)
+        smp = AG.integrate(sim, {'c:a': Gt, 'm:eta': et}, x0, T, WIN,
+                           seed=SEED)
+        L = loss_fn(smp, fct)
+        L.sum().backward()
+        out.append((L.detach().double().cpu().numpy(),
+                    Gt.grad.cpu().numpy(), et.grad.cpu().numpy()))
+    np.testing.assert_allclose(out[1][0], out[0][0], rtol=0, atol=1e-6)
+    for a, b in zip(out[0][1:], out[1][1:]):
+        np.testing.assert_allclose(b, a, rtol=0,
+                                   atol=1e-4 * np.abs(a).max())
+
+
 def test_adjoint_grouped_weights_matches_device_finite_differences():
     """Two delay groups of 64 instances with their own weights and delays
     (subjects on one sparsity pattern): the reverse sweep reads each group's

@@ -140,6 +140,10 @@ struct tvbx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipDeviceProp_t prop{};
+  // device workspace of the context-level calls (tvbx_fc_loss), grown on
+  // demand and kept until tvbx_ctx_destroy
+  void *work = nullptr;
+  size_t work_bytes = 0;
 };
 
 struct tvbx_module {
@@ -319,6 +323,7 @@ extern "C" int tvbx_ctx_destroy(tvbx_ctx *c) {
     hipStreamSynchronize(c->stream);
     hipStreamDestroy(c->stream);
   }
+  if (c->work) hipFree(c->work);
   delete c;
   return TVBX_OK;
 }
@@ -462,6 +467,18 @@ __global__ void k_fill_f64(double *p, size_t n, double v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) p[i] = v;
+}
+
+// out[r][p] = in[r][p] (pitch-strided FP64 rows -> compact float32 rows)
+__global__ void k_f64_to_f32_rows(const double *__restrict__ in, size_t rows,
+                                  int pitch, int P, float *__restrict__ out) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; k < rows * (size_t)P; k += stride) {
+    const size_t r = k / P;
+    const int p = (int)(k - r * P);
+    out[k] = (float)in[r * pitch + p];
+  }
 }
 
 __global__ void k_fill_u32(unsigned *p, size_t n, unsigned v) {
@@ -2155,6 +2172,245 @@ extern "C" int tvbx_corr_noalloc(tvbx_ctx *c, const float *samples,
                      pitch, (int)n, 0.0, u, mean);
   hipLaunchKernelGGL(k_fc_corr_full, dim3((P + 63) / 64, N), dim3(64), 0, st,
                      m2, N, pitch, 0, P, corr);
+  HIPCHK(hipGetLastError());
+  return TVBX_OK;
+}
+
+// ----------------------------------------------------------- FC-fit loss
+// The loss of the FC-fit inversion, per instance p,
+//   L_p = 1 - r(a, b),  a_ij = M2_ij / sqrt(M2_ii M2_jj),  b_ij = target_ij,
+// r the Pearson correlation of the upper triangles (i < j), with its
+// derivative with respect to the FP64 co-moments M2 of the samples.  With
+// a^ and b^ the centred triangles, dr/da = b^ / (|a^| |b^|) - r a^ / |a^|^2
+// (both terms already centred), G = dL/da = -dr/da, and through
+// a_ij = M2_ij rs_i rs_j (rs_i = 1 / sqrt(M2_ii)):
+//   dL/dM2_ij = G_ij rs_i rs_j (i < j),
+//   dL/dM2_kk = -1/2 rs_k^2 D_k,   D_k = sum over j != k of G_kj a_kj.
+// Since M2_ij = sum_m xc_mi xc_mj (xc = x - mean, sum_m xc_m = 0), the
+// samples' gradient is dL/dx_mk = sum_j S_kj xc_mj with S symmetric,
+// S_kj = dL/dM2_kj (k != j) and S_kk = 2 dL/dM2_kk = -rs_k^2 D_k.
+//   One lane per instance; the waves of a workgroup share its 64 instances
+// and take the rows k = w, w + NW, ...; their partial sums meet in LDS.  A
+// pair's a_ij and G_ij are computed from (min, max) in one order, so the two
+// waves that need a pair get the same values and S comes out symmetric.
+#define FL_NW 16
+__device__ __forceinline__ double fl_a(const double *__restrict__ m2,
+                                       const double *__restrict__ rs, int i,
+                                       int j, int N, int pitch, int p) {
+  const int lo = min(i, j), hi = max(i, j);
+  return m2[tri_index(lo, hi, N) * pitch + p] * rs[(size_t)lo * pitch + p] *
+         rs[(size_t)hi * pitch + p];
+}
+
+__device__ __forceinline__ double fl_b(const float *__restrict__ target,
+                                       int per_inst, int i, int j, int N,
+                                       int P, int p) {
+  const int lo = min(i, j), hi = max(i, j);
+  return per_inst ? (double)target[((size_t)lo * N + hi) * P + p]
+                  : (double)target[(size_t)lo * N + hi];
+}
+
+// sum over the workgroup's waves of v (one value per lane), for every wave
+__device__ __forceinline__ double fl_wg_sum(double v, double *red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  red[w * 64 + lane] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int k = 0; k < FL_NW; ++k) t += red[k * 64 + lane];
+  __syncthreads();
+  return t;
+}
+
+// grid ceil(P / 64), block 64 * FL_NW.  m2 [N(N+1)/2][pitch], target
+// [N][N] (shared) or [N][N][P] (per instance); rs [N][pitch] workspace;
+// loss [P]; s [N][N][P].
+__global__ void __launch_bounds__(64 * FL_NW)
+k_fl_loss(const double *__restrict__ m2, int N, int pitch, int P,
+          const float *__restrict__ target, int per_inst,
+          double *__restrict__ rs, float *__restrict__ loss,
+          float *__restrict__ s) {
+  __shared__ double red[64 * FL_NW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int p = blockIdx.x * 64 + lane;
+  const bool on = p < P;
+  // lanes past P compute on the workgroup's first instance (always < P, and
+  // its rs column is written by this workgroup) and store nothing
+  const int pp = on ? p : blockIdx.x * 64;
+  for (int i = w; i < N; i += FL_NW)
+    rs[(size_t)i * pitch + p] =
+        1.0 / sqrt(m2[tri_index(i, i, N) * pitch + pp]);
+  __syncthreads();
+  const double *rsp = rs;
+  // pass 1: means of the two triangles
+  double sa = 0.0, sb = 0.0;
+  for (int i = w; i < N; i += FL_NW)
+    for (int j = i + 1; j < N; ++j) {
+      sa += fl_a(m2, rsp, i, j, N, pitch, pp);
+      sb += fl_b(target, per_inst, i, j, N, P, pp);
+    }
+  const double K = 0.5 * (double)N * (double)(N - 1);
+  const double ma = fl_wg_sum(sa, red) / K, mb = fl_wg_sum(sb, red) / K;
+  // pass 2: norms and the dot product of the centred triangles
+  double saa = 0.0, sbb = 0.0, sab = 0.0;
+  for (int i = w; i < N; i += FL_NW)
+    for (int j = i + 1; j < N; ++j) {
+      const double a = fl_a(m2, rsp, i, j, N, pitch, pp) - ma;
+      const double b = fl_b(target, per_inst, i, j, N, P, pp) - mb;
+      saa = fma(a, a, saa);
+      sbb = fma(b, b, sbb);
+      sab = fma(a, b, sab);
+    }
+  saa = fl_wg_sum(saa, red);
+  sbb = fl_wg_sum(sbb, red);
+  sab = fl_wg_sum(sab, red);
+  const double nab = sqrt(saa * sbb);
+  const double r = sab / nab;
+  if (on && w == 0) loss[p] = (float)(1.0 - r);
+  // pass 3: S, row k by its wave
+  for (int k = w; k < N; k += FL_NW) {
+    const double rk = rsp[(size_t)k * pitch + pp];
+    double D = 0.0;
+    for (int j = 0; j < N; ++j) {
+      if (j == k) continue;
+      const double a = fl_a(m2, rsp, k, j, N, pitch, pp);
+      const double b = fl_b(target, per_inst, k, j, N, P, pp);
+      const double G = -((b - mb) / nab - r * (a - ma) / saa);
+      D = fma(G, a, D);
+      if (on)
+        s[((size_t)k * N + j) * P + p] =
+            (float)(G * rk * rsp[(size_t)j * pitch + pp]);
+    }
+    if (on) s[((size_t)k * N + k) * P + p] = (float)(-rk * rk * D);
+  }
+}
+
+// out[c][r] = in[r][c] - sub[r % sub_rows][c] (sub null: no subtraction),
+// in [rows][cols] -> out [cols][rows], through 64 x 64 tiles in LDS.
+// grid (ceil(cols / 64), ceil(rows / 64)) (the y extent split over z when
+// it passes 65535), block 256.
+__global__ void __launch_bounds__(256)
+k_transpose(const float *__restrict__ in, long long rows, long long cols,
+            const float *__restrict__ sub, int sub_rows,
+            float *__restrict__ out) {
+  __shared__ float tile[64][65];
+  const long long c0 = (long long)blockIdx.x * 64;
+  const long long r0 = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * 64;
+  if (r0 >= rows) return;  // whole workgroup
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int k = ty; k < 64; k += 4) {
+    const long long r = r0 + k, c = c0 + tx;
+    float v = 0.0f;
+    if (r < rows && c < cols) {
+      v = in[r * cols + c];
+      if (sub) v -= sub[(r % sub_rows) * cols + c];
+    }
+    tile[k][tx] = v;
+  }
+  __syncthreads();
+  for (int k = ty; k < 64; k += 4) {
+    const long long c = c0 + k, r = r0 + tx;
+    if (c < cols && r < rows) out[c * rows + r] = tile[tx][k];
+  }
+}
+
+static int ctx_work(tvbx_ctx *c, size_t bytes) {
+  if (c->work && c->work_bytes >= bytes) return TVBX_OK;
+  if (c->work) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(c->work));
+  }
+  c->work = nullptr;
+  c->work_bytes = 0;
+  HIPCHK(hipMalloc(&c->work, bytes));
+  c->work_bytes = bytes;
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_fc_loss(tvbx_ctx *c, const float *samples, long long n,
+                            int N, int P, const float *target,
+                            int target_per_instance, float *loss, float *s,
+                            float *mean) {
+  CHECK(c && samples && target && loss && s && mean, "null argument");
+  CHECK(n >= 2 && N >= 3 && P >= 1,
+        "need n >= 2 samples, N >= 3 nodes (two pairs), P >= 1");
+  CHECK(n <= (1ll << 30), "too many samples");
+  HIPCHK(hipSetDevice(c->device));
+  const int pitch = (P + 63) / 64 * 64;
+  const size_t NP = (size_t)N * pitch;
+  const size_t tri = (size_t)N * (N + 1) / 2;
+  const int T = (N + FC_TILE - 1) / FC_TILE;
+  const int n_tiles = T * (T + 1) / 2;
+  const int n_fblk = pitch / FC_PB;
+  const bool stage = pitch != P;  // the FC kernels read [n][N][pitch]
+  size_t used = 0;
+  auto take = [&used](size_t bytes) {
+    const size_t o = used;
+    used += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t o_m2 = take(tri * pitch * 8), o_mean = take(NP * 8),
+               o_s1 = take(NP * 8), o_u = take(NP * 8),
+               o_shift = take(NP * 4), o_rs = take(NP * 8),
+               o_tiles = take((size_t)n_tiles * 8),
+               o_smp = stage ? take((size_t)n * NP * 4) : 0;
+  int rc = ctx_work(c, used);
+  if (rc != TVBX_OK) return rc;
+  char *base = static_cast<char *>(c->work);
+  double *m2 = reinterpret_cast<double *>(base + o_m2);
+  double *dmean = reinterpret_cast<double *>(base + o_mean);
+  double *s1 = reinterpret_cast<double *>(base + o_s1);
+  double *u = reinterpret_cast<double *>(base + o_u);
+  float *shift = reinterpret_cast<float *>(base + o_shift);
+  double *rs = reinterpret_cast<double *>(base + o_rs);
+  int2 *tiles = reinterpret_cast<int2 *>(base + o_tiles);
+  const unsigned long long nwg =
+      ((unsigned long long)n_tiles * n_fblk + 7) / 8 * 8;
+  CHECK(nwg < (1ull << 31), "FC problem too large for one launch");
+  hipStream_t st = c->stream;
+  const float *smp = samples;
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid1d(tri * pitch, 256)), dim3(256),
+                     0, st, m2, tri * pitch, 0.0);
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid1d(NP, 256)), dim3(256), 0, st,
+                     dmean, NP, 0.0);
+  hipLaunchKernelGGL(k_fc_tile_table, dim3(1), dim3(64), 0, st, T, tiles);
+  if (stage) {
+    float *sp = reinterpret_cast<float *>(base + o_smp);
+    hipLaunchKernelGGL(k_pitch_rows, dim3(grid1d((size_t)n * NP, 256)),
+                       dim3(256), 0, st, samples, (size_t)n * N, P, pitch,
+                       sp);
+    smp = sp;
+  }
+  hipLaunchKernelGGL(k_fc_prep, dim3(pitch / 64, N), dim3(64), 0, st, smp,
+                     (int)n, 0ll, (int)n, N, pitch, 0.0, dmean, shift, s1, u);
+  hipLaunchKernelGGL(k_fc_update, dim3((unsigned)nwg), dim3(64 * FC_NW), 0,
+                     st, smp, (int)n, 0ll, (int)n, N, pitch, 0.0, tiles,
+                     n_tiles, n_fblk, shift, s1, u, m2);
+  hipLaunchKernelGGL(k_fc_mean, dim3(pitch / 64, N), dim3(64), 0, st, N,
+                     pitch, (int)n, 0.0, u, dmean);
+  hipLaunchKernelGGL(k_fl_loss, dim3(pitch / 64), dim3(64 * FL_NW), 0, st, m2,
+                     N, pitch, P, target, target_per_instance, rs, loss, s);
+  hipLaunchKernelGGL(k_f64_to_f32_rows, dim3(grid1d((size_t)N * P, 256)),
+                     dim3(256), 0, st, dmean, (size_t)N, pitch, P, mean);
+  HIPCHK(hipGetLastError());
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_transpose(tvbx_ctx *c, const float *in, long long rows,
+                              long long cols, const float *sub, int sub_rows,
+                              float *out) {
+  CHECK(c && in && out, "null argument");
+  CHECK(rows >= 1 && cols >= 1, "empty matrix");
+  CHECK(!sub || sub_rows >= 1, "sub_rows must be >= 1 with sub");
+  HIPCHK(hipSetDevice(c->device));
+  const long long gx = (cols + 63) / 64, gr = (rows + 63) / 64;
+  CHECK(gx <= 0x7fffffffLL, "too many columns");
+  const long long gy = std::min<long long>(gr, 65535);
+  const long long gz = (gr + gy - 1) / gy;
+  CHECK(gz <= 65535, "too many rows");
+  hipLaunchKernelGGL(k_transpose, dim3((unsigned)gx, (unsigned)gy,
+                                       (unsigned)gz),
+                     dim3(256), 0, c->stream, in, rows, cols, sub,
+                     sub ? sub_rows : 1, out);
   HIPCHK(hipGetLastError());
   return TVBX_OK;
 }

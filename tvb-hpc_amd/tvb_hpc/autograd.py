@@ -140,3 +140,66 @@ def fc_loss(samples, fc_target):
     b = b - b.mean(dim=1, keepdim=True)
     r = (a * b).sum(dim=1) / torch.sqrt((a * a).sum(dim=1) * (b * b).sum(dim=1))
     return 1.0 - r
+
+
+class _FcLossDevice(torch.autograd.Function):
+    """fc_loss of device samples on the HIP kernels: the FP64 co-moments
+    and the loss with its derivative in libtvbx (tvbx_fc_loss), the samples'
+    gradient as one batched float32 product per instance,
+    dL_p/dx_p = gout_p (x_p - mean_p) S_p, between two transposes
+    (tvbx_transpose) that take the instance axis outermost and back."""
+
+    @staticmethod
+    def forward(ctx, samples, fc_target, hctx):
+        x = samples.detach().float().contiguous()
+        M, N, P = x.shape
+        dev = x.device
+        tgt = torch.as_tensor(fc_target, device=dev).float()
+        per = tgt.ndim == 3
+        # the kernel reads a per-instance target instance-innermost
+        tgt = (tgt.permute(1, 2, 0) if per else tgt).contiguous()
+        loss = torch.empty(P, dtype=torch.float32, device=dev)
+        s = torch.empty((N, N, P), dtype=torch.float32, device=dev)
+        mean = torch.empty((N, P), dtype=torch.float32, device=dev)
+        # torch's stream has written the inputs before the context's reads
+        torch.cuda.current_stream(dev).synchronize()
+        hctx.fc_loss(x.data_ptr(), M, N, P, tgt.data_ptr(), per,
+                     loss.data_ptr(), s.data_ptr(), mean.data_ptr())
+        hctx.sync()
+        ctx.save_for_backward(x, s, mean)
+        ctx.hctx = hctx
+        ctx.dtype = samples.dtype
+        return loss.to(samples.dtype)
+
+    @staticmethod
+    def backward(ctx, gout):
+        x, s, mean = ctx.saved_tensors
+        M, N, P = x.shape
+        hctx = ctx.hctx
+        dev = x.device
+        xt = torch.empty((P, M * N), dtype=torch.float32, device=dev)
+        sp = (s.permute(2, 0, 1) *
+              gout.detach().float()[:, None, None]).contiguous()
+        torch.cuda.current_stream(dev).synchronize()
+        hctx.transpose(x.data_ptr(), M * N, P, xt.data_ptr(),
+                       mean.data_ptr(), N)
+        hctx.sync()
+        y = torch.bmm(xt.view(P, M, N), sp)
+        dx = torch.empty((M, N, P), dtype=torch.float32, device=dev)
+        torch.cuda.current_stream(dev).synchronize()
+        hctx.transpose(y.data_ptr(), P, M * N, dx.data_ptr())
+        hctx.sync()
+        return dx.to(ctx.dtype), None, None
+
+
+def fc_loss_device(samples, fc_target, context=None):
+    """fc_loss (above) of samples (M, N, P) on the device, FP64 co-moments
+    and loss in libtvbx, float32 gradient; fc_target (N, N) or (P, N, N).
+    Differentiable with respect to ``samples``."""
+    if not samples.is_cuda:
+        raise ValueError('fc_loss_device needs device samples; fc_loss '
+                         'takes host tensors')
+    from .hip import Context
+    hctx = context or Context.default(samples.device.index or 0)
+    return _FcLossDevice.apply(samples, fc_target, hctx)
+

@@ -116,6 +116,8 @@ _SIGS = {
     'tvbx_rng_normals': [_P, ctypes.c_ulonglong, _P, _SZ],
     'tvbx_corr': [_P, _P, _LL, _I, _I, _P],
     'tvbx_corr_noalloc': [_P, _P, _LL, _I, _I, _P],
+    'tvbx_fc_loss': [_P, _P, _LL, _I, _I, _P, _I, _P, _P, _P],
+    'tvbx_transpose': [_P, _P, _LL, _LL, _P, _I, _P],
     'tvbx_rmap_create': [_P, _P, _I, _I, _P],
     'tvbx_rmap_destroy': [_P],
     'tvbx_rmap_to_avg': [_P, _I, _I, _P, _P],
@@ -277,6 +279,31 @@ class Context:
         check(lib().tvbx_corr_noalloc(self.h, ctypes.c_void_p(samples_ptr),
                                       int(n), int(N), int(P),
                                       ctypes.c_void_p(corr_ptr)))
+
+    def fc_loss(self, samples_ptr, n, N, P, target_ptr, per_instance,
+                loss_ptr, s_ptr, mean_ptr):
+        """tvbx_fc_loss on device memory (addresses, e.g. torch tensors'
+        data_ptr()): loss (P,), the loss's symmetric derivative matrix s
+        (N, N, P) and the means (N, P) of samples (n, N, P) against target
+        (N, N), or (N, N, P) when ``per_instance``; queued on this context's
+        stream."""
+        check(lib().tvbx_fc_loss(self.h, ctypes.c_void_p(samples_ptr),
+                                 int(n), int(N), int(P),
+                                 ctypes.c_void_p(target_ptr),
+                                 int(bool(per_instance)),
+                                 ctypes.c_void_p(loss_ptr),
+                                 ctypes.c_void_p(s_ptr),
+                                 ctypes.c_void_p(mean_ptr)))
+
+    def transpose(self, in_ptr, rows, cols, out_ptr, sub_ptr=None,
+                  sub_rows=1):
+        """out (cols, rows) = in (rows, cols) transposed, minus
+        sub[r % sub_rows] (cols,) per row r when sub_ptr is given; float32
+        device memory, queued on this context's stream."""
+        check(lib().tvbx_transpose(self.h, ctypes.c_void_p(in_ptr),
+                                   int(rows), int(cols),
+                                   ctypes.c_void_p(sub_ptr or 0),
+                                   int(sub_rows), ctypes.c_void_p(out_ptr)))
 
 
 class RegionMap:
