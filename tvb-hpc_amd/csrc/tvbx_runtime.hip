@@ -217,6 +217,12 @@ struct tvbx_sim {
   bool ck_recorded = false;
   DevBuf lam, mu, rec_x, rec_sum, forcing, g_c, g_m, g_out, save_state,
       save_ring;
+  // the adjoint's replay of window k - 1 runs on a second stream while the
+  // reverse sweep of window k runs on the context's: two sets of recordings
+  // and one event per window and side
+  DevBuf rec_x2, rec_sum2;
+  hipStream_t aux = nullptr;
+  std::vector<hipEvent_t> adj_ev;
   // pacing of blocked launches (tvbx_sim_set_pacing): 0 off, else the lag
   int pace = 0;
   DevBuf pace_buf;
@@ -1049,8 +1055,13 @@ extern "C" int tvbx_sim_destroy(tvbx_sim *s) {
   dfree(s->clock);
   DevBuf *adj[] = {&s->ccol, &s->crow, &s->cw, &s->lam, &s->mu, &s->rec_x,
                    &s->rec_sum, &s->forcing, &s->g_c, &s->g_m, &s->g_out,
-                   &s->save_state, &s->save_ring};
+                   &s->save_state, &s->save_ring, &s->rec_x2, &s->rec_sum2};
   for (DevBuf *b : adj) dfree(*b);
+  if (s->aux) {
+    hipStreamSynchronize(s->aux);
+    hipStreamDestroy(s->aux);
+  }
+  for (hipEvent_t e : s->adj_ev) hipEventDestroy(e);
   DevBuf *mf[] = {&s->mfma_blk, &s->mfma_pairs, &s->mfma_w, &s->csum,
                   &s->pace_buf};
   for (DevBuf *b : mf) dfree(*b);
@@ -2790,30 +2801,77 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
   // most d_min >= steps_per_launch - 1 steps
   const bool blocked = spl > 1 && d.node_block < N;
   const int spl_rev = blocked ? std::max(1, spl - 1) : spl;
-  for (long long kw = K - 1; kw >= 0; --kw) {
-    const long long n_lo = s->ck_step0 + kw * W;
-    // forward replay of the window, recording x(n) and sum(n), unless the
-    // forward run recorded them
-    if (!s->ck_recorded) {
-      HIPCHK(hipMemcpyAsync(s->state.p, s->ck_state[kw].p, s->state.bytes,
-                            hipMemcpyDeviceToDevice, s->ctx->stream));
-      HIPCHK(hipMemcpyAsync(s->ring.p, s->ck_ring[kw].p, s->ring.bytes,
-                            hipMemcpyDeviceToDevice, s->ctx->stream));
+  // Windows are swept from the last.  Replaying window k - 1 (restore its
+  // checkpoint, run it forward recording x(n) and sum(n)) needs nothing the
+  // reverse sweep of window k touches: the sweep reads the recordings of
+  // window k, lambda, mu and the parameters, never the state or the ring.
+  // So the replay runs on a second stream, into the other of two recording
+  // buffers, while the context's stream sweeps window k; events order each
+  // sweep after its window's replay and each replay after the sweep that
+  // last read its buffer.
+  const bool replay = !s->ck_recorded;
+  const bool overlap = replay && K > 1;
+  hipStream_t sa = s->ctx->stream, sb = sa;
+  if (overlap) {
+    rc = dalloc(s->rec_x2, (size_t)W * S * NP * 4);
+    if (rc == TVBX_OK) rc = dalloc(s->rec_sum2, (size_t)W * NP * 4);
+    if (rc != TVBX_OK) return rc;
+    if (!s->aux) HIPCHK(hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
+    sb = s->aux;
+    while ((long long)s->adj_ev.size() < 2 * K + 1) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      s->adj_ev.push_back(e);
     }
-    for (long long k = 0; !s->ck_recorded && k < W; k += spl) {
+    // the replays overwrite the state and ring the context's stream saved
+    HIPCHK(hipEventRecord(s->adj_ev[2 * K], sa));
+    HIPCHK(hipStreamWaitEvent(sb, s->adj_ev[2 * K], 0));
+  }
+  auto rec_x_of = [&](long long kw) {
+    return (float *)((overlap && (kw & 1)) ? s->rec_x2.p : s->rec_x.p);
+  };
+  auto rec_sum_of = [&](long long kw) {
+    return (float *)((overlap && (kw & 1)) ? s->rec_sum2.p : s->rec_sum.p);
+  };
+  auto ev_rep = [&](long long kw) { return s->adj_ev[2 * kw]; };
+  auto ev_rev = [&](long long kw) { return s->adj_ev[2 * kw + 1]; };
+  // forward replay of window kw on stream sb
+  auto issue_replay = [&](long long kw) -> int {
+    const long long n_lo = s->ck_step0 + kw * W;
+    HIPCHK(hipMemcpyAsync(s->state.p, s->ck_state[kw].p, s->state.bytes,
+                          hipMemcpyDeviceToDevice, sb));
+    HIPCHK(hipMemcpyAsync(s->ring.p, s->ck_ring[kw].p, s->ring.bytes,
+                          hipMemcpyDeviceToDevice, sb));
+    for (long long k = 0; k < W; k += spl) {
       tvbx_step_args a;
       s->step = n_lo + k;
       fill_args(s, a);
       a.n_steps = (int)std::min<long long>(spl, W - k);
       a.tavg_period = 0;
       a.out_input = nullptr;
-      a.rec_x = (float *)s->rec_x.p;
-      a.rec_sum = (float *)s->rec_sum.p;
+      a.rec_x = rec_x_of(kw);
+      a.rec_sum = rec_sum_of(kw);
       a.rec_step0 = n_lo;
       void *params[] = {&a};
-      HIPCHK(hipModuleLaunchKernel(step_fn(s), gx, gy, 1, bx, 1, 1, 0,
-                                   s->ctx->stream, params, nullptr));
+      HIPCHK(hipModuleLaunchKernel(step_fn(s), gx, gy, 1, bx, 1, 1, 0, sb,
+                                   params, nullptr));
     }
+    if (overlap) HIPCHK(hipEventRecord(ev_rep(kw), sb));
+    return TVBX_OK;
+  };
+  if (replay) {
+    rc = issue_replay(K - 1);
+    if (rc != TVBX_OK) return rc;
+  }
+  for (long long kw = K - 1; kw >= 0; --kw) {
+    const long long n_lo = s->ck_step0 + kw * W;
+    if (overlap && kw >= 1) {
+      // window kw - 1 records into the buffer the sweep of kw + 1 read
+      if (kw + 1 < K) HIPCHK(hipStreamWaitEvent(sb, ev_rev(kw + 1), 0));
+      rc = issue_replay(kw - 1);
+      if (rc != TVBX_OK) return rc;
+    }
+    if (overlap) HIPCHK(hipStreamWaitEvent(sa, ev_rep(kw), 0));
     // reverse sweep of the window
     for (long long k = 0; k < W; k += spl_rev) {
       tvbx_adj_args A;
@@ -2826,8 +2884,8 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
       A.ccol = (const int *)s->ccol.p;
       A.crow = (const unsigned *)s->crow.p;
       A.cw = (const float *)s->cw.p;
-      A.rec_x = (const float *)s->rec_x.p;
-      A.rec_sum = (const float *)s->rec_sum.p;
+      A.rec_x = rec_x_of(kw);
+      A.rec_sum = rec_sum_of(kw);
       A.lam = (float *)s->lam.p;
       A.mu = (float *)s->mu.p;
       A.forcing = n_samples ? (const float *)s->forcing.p : nullptr;
@@ -2836,9 +2894,10 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
       A.g_c = (float *)s->g_c.p;
       A.g_m = (float *)s->g_m.p;
       void *params[] = {&A};
-      HIPCHK(hipModuleLaunchKernel(s->mod->adj, gx, gy, 1, bx, 1, 1, 0,
-                                   s->ctx->stream, params, nullptr));
+      HIPCHK(hipModuleLaunchKernel(s->mod->adj, gx, gy, 1, bx, 1, 1, 0, sa,
+                                   params, nullptr));
     }
+    if (overlap) HIPCHK(hipEventRecord(ev_rev(kw), sa));
   }
   // restore the end-of-run state
   HIPCHK(hipMemcpyAsync(s->state.p, s->save_state.p, s->state.bytes,
