@@ -287,6 +287,14 @@ int tvbx_sim_adjoint(tvbx_sim *sim, const float *dsamples, int on_device,
                      long long sample0, long long n_samples, int n_gc,
                      float *g_coupling, int n_gm, float *g_model,
                      int out_on_device);
+/* Whether tvbx_sim_adjoint keeps the checkpoints (no counterpart in the
+ * reference, which has no gradient): keep = 1 (the default) restores each
+ * window's checkpoint by a copy before replaying it and leaves them all for
+ * another adjoint of the same run; keep = 0 replays each window inside its
+ * checkpoint's own buffers (no restore copies, no copy of the end-of-run
+ * history), which consumes them: a second tvbx_sim_adjoint then fails until
+ * tvbx_sim_checkpoint_run runs again.  Gradients are the same either way. */
+int tvbx_sim_set_checkpoint_reuse(tvbx_sim *sim, int keep);
 
 /* ---- RNG ------------------------------------------------------------------
  * Reference: RNG.fill (tvb_hpc/rng.py:63-96) fills a float32 array with

@@ -157,6 +157,47 @@ def test_adjoint_repeatable_and_state_restored():
     assert sim.step == T
 
 
+def test_adjoint_consuming_checkpoints_gives_the_same_gradients():
+    """With checkpoint reuse off the adjoint replays each window inside its
+    checkpoint's buffers: the gradients and the end-of-run state equal those
+    of the copying adjoint, and a second adjoint refuses to run on the
+    consumed checkpoints until checkpoint_run runs again."""
+    from tvb_hpc.hip import TvbxError
+    csr, m, c, sc, G, eta, x0 = _setup()
+    out = []
+    for keep in (True, False):
+        sim = _device_sim(csr, m, c, sc, G, eta)
+        sim.set_checkpoint_reuse(keep)
+        sim.set_state(x0)
+        sim.init_history('zeros')
+        sim.set_noise(SEED)
+        sim.checkpoint_run(T, WIN)
+        x_end = sim.get_state()
+        smp, first = sim.samples()
+        dS = np.random.default_rng(0).standard_normal(smp.shape).astype(
+            np.float32)
+        g = sim.adjoint(dS, first)
+        np.testing.assert_array_equal(sim.get_state(), x_end)
+        assert sim.step == T
+        if not keep:
+            with pytest.raises(TvbxError, match='consumed'):
+                sim.adjoint(dS, first)
+            # a new checkpointed run makes it usable again
+            sim.set_state(x0)
+            sim.step = 0
+            sim.init_history('zeros')
+            sim.checkpoint_run(T, WIN)
+            g2 = sim.adjoint(dS, first)
+            for k in g:
+                np.testing.assert_array_equal(g[k], g2[k])
+        # the simulation continues from the end of the run
+        sim.run(WIN)
+        out.append((g, sim.get_state()))
+    for k in out[0][0]:
+        np.testing.assert_array_equal(out[0][0][k], out[1][0][k])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
 def test_adjoint_with_temporal_blocking():
     """Forward replay in blocked launches and reverse launches of several
     steps give the same samples and gradients, bit for bit, as one-step

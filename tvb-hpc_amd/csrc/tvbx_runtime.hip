@@ -211,6 +211,8 @@ struct tvbx_sim {
   std::vector<DevBuf> ck_state, ck_ring;
   long long ck_step0 = -1, ck_T = 0;
   int ck_W = 0;
+  bool ck_keep = true;       // tvbx_sim_set_checkpoint_reuse
+  bool ck_consumed = false;  // an adjoint replayed inside the checkpoints
   // checkpoint_run with window 0: x(n) and sum(n) of every step recorded
   // during the forward run, so the adjoint needs no replay
   bool rec_forward = false;
@@ -1258,6 +1260,12 @@ extern "C" int tvbx_sim_set_pacing(tvbx_sim *s, int on) {
   }
   if (on != s->pace) drop_graphs(s);
   s->pace = on;
+  return TVBX_OK;
+}
+
+extern "C" int tvbx_sim_set_checkpoint_reuse(tvbx_sim *s, int keep) {
+  CHECK(s && (keep == 0 || keep == 1), "bad argument");
+  s->ck_keep = keep != 0;
   return TVBX_OK;
 }
 
@@ -2688,6 +2696,7 @@ extern "C" int tvbx_sim_checkpoint_run(tvbx_sim *s, long long n_steps,
   const tvbx_sim_desc &d = s->d;
   HIPCHK(hipSetDevice(s->ctx->device));
   s->ck_recorded = false;
+  s->ck_consumed = false;
   if (window == 0) {
     // record the whole run: no checkpoints, no replay in the adjoint
     CHECK(n_steps <= 0x7fffffffLL, "too many steps to record");
@@ -2705,6 +2714,7 @@ extern "C" int tvbx_sim_checkpoint_run(tvbx_sim *s, long long n_steps,
     s->ck_recorded = true;
     return TVBX_OK;
   }
+  s->ck_consumed = false;
   const long long K = n_steps / window;
   const size_t sb = s->state.bytes, rb = s->ring.bytes;
   if ((long long)s->ck_state.size() != K) {
@@ -2745,6 +2755,9 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
   int rc = adjoint_checks(s);
   if (rc != TVBX_OK) return rc;
   CHECK(s->ck_T > 0, "run tvbx_sim_checkpoint_run first");
+  CHECK(!s->ck_consumed,
+        "the previous adjoint consumed the checkpoints (checkpoint reuse "
+        "off): run tvbx_sim_checkpoint_run again");
   CHECK(n_samples >= 0 && (n_samples == 0 || dsamples), "bad forcing");
   const tvbx_sim_desc &d = s->d;
   HIPCHK(hipSetDevice(s->ctx->device));
@@ -2768,8 +2781,13 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
   AL(g_c, (size_t)ngc * NP * 4);
   AL(g_m, (size_t)ngm * NP * 4);
   AL(g_out, (size_t)ngc * pitch * 4);
-  AL(save_state, s->state.bytes);
-  AL(save_ring, s->ring.bytes);
+  // replaying inside the checkpoints leaves the end-of-run state and ring
+  // untouched: nothing to save
+  const bool consume = !s->ck_recorded && !s->ck_keep;
+  if (!consume) {
+    AL(save_state, s->state.bytes);
+    AL(save_ring, s->ring.bytes);
+  }
 #undef AL
   if (r2 != TVBX_OK) return r2;
   DevBuf *z[] = {&s->lam, &s->mu, &s->forcing, &s->g_c, &s->g_m};
@@ -2788,10 +2806,12 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
   }
   // keep the end-of-run state so the simulation can continue afterwards
   const long long step_end = s->step;
-  HIPCHK(hipMemcpyAsync(s->save_state.p, s->state.p, s->state.bytes,
-                        hipMemcpyDeviceToDevice, s->ctx->stream));
-  HIPCHK(hipMemcpyAsync(s->save_ring.p, s->ring.p, s->ring.bytes,
-                        hipMemcpyDeviceToDevice, s->ctx->stream));
+  if (!consume) {
+    HIPCHK(hipMemcpyAsync(s->save_state.p, s->state.p, s->state.bytes,
+                          hipMemcpyDeviceToDevice, s->ctx->stream));
+    HIPCHK(hipMemcpyAsync(s->save_ring.p, s->ring.p, s->ring.bytes,
+                          hipMemcpyDeviceToDevice, s->ctx->stream));
+  }
   const unsigned gx = pitch / (64 * d.ipl);
   const unsigned gy = (N + d.node_block - 1) / d.node_block;
   const unsigned bx = 64u * d.waves;
@@ -2835,17 +2855,29 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
   };
   auto ev_rep = [&](long long kw) { return s->adj_ev[2 * kw]; };
   auto ev_rev = [&](long long kw) { return s->adj_ev[2 * kw + 1]; };
-  // forward replay of window kw on stream sb
+  // forward replay of window kw on stream sb: in copies of its checkpoint
+  // (the simulation's own state and ring), or, consuming the checkpoint, in
+  // the checkpoint's buffers themselves (the launches take those addresses;
+  // the simulation's buffers keep the end of the run)
+  void *const state_run = s->state.p, *const ring_run = s->ring.p;
   auto issue_replay = [&](long long kw) -> int {
     const long long n_lo = s->ck_step0 + kw * W;
-    HIPCHK(hipMemcpyAsync(s->state.p, s->ck_state[kw].p, s->state.bytes,
-                          hipMemcpyDeviceToDevice, sb));
-    HIPCHK(hipMemcpyAsync(s->ring.p, s->ck_ring[kw].p, s->ring.bytes,
-                          hipMemcpyDeviceToDevice, sb));
+    if (!consume) {
+      HIPCHK(hipMemcpyAsync(s->state.p, s->ck_state[kw].p, s->state.bytes,
+                            hipMemcpyDeviceToDevice, sb));
+      HIPCHK(hipMemcpyAsync(s->ring.p, s->ck_ring[kw].p, s->ring.bytes,
+                            hipMemcpyDeviceToDevice, sb));
+    }
     for (long long k = 0; k < W; k += spl) {
       tvbx_step_args a;
       s->step = n_lo + k;
+      if (consume) {
+        s->state.p = s->ck_state[kw].p;
+        s->ring.p = s->ck_ring[kw].p;
+      }
       fill_args(s, a);
+      s->state.p = state_run;
+      s->ring.p = ring_run;
       a.n_steps = (int)std::min<long long>(spl, W - k);
       a.tavg_period = 0;
       a.out_input = nullptr;
@@ -2900,10 +2932,13 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
     if (overlap) HIPCHK(hipEventRecord(ev_rev(kw), sa));
   }
   // restore the end-of-run state
-  HIPCHK(hipMemcpyAsync(s->state.p, s->save_state.p, s->state.bytes,
-                        hipMemcpyDeviceToDevice, s->ctx->stream));
-  HIPCHK(hipMemcpyAsync(s->ring.p, s->save_ring.p, s->ring.bytes,
-                        hipMemcpyDeviceToDevice, s->ctx->stream));
+  if (!consume) {
+    HIPCHK(hipMemcpyAsync(s->state.p, s->save_state.p, s->state.bytes,
+                          hipMemcpyDeviceToDevice, s->ctx->stream));
+    HIPCHK(hipMemcpyAsync(s->ring.p, s->save_ring.p, s->ring.bytes,
+                          hipMemcpyDeviceToDevice, s->ctx->stream));
+  }
+  s->ck_consumed = consume;
   s->step = step_end;
   s->clock_dirty = true;
   // outputs: coupling gradients summed over nodes [n_gc][P]; model

@@ -32,6 +32,8 @@ class _Integrate(torch.autograd.Function):
         sim.step = 0
         sim.init_history('zeros')
         sim.set_noise(seed)
+        # one backward pass per forward: it may replay inside the checkpoints
+        sim.set_checkpoint_reuse(False)
         sim.checkpoint_run(n_steps, window)
         ctx.sim, ctx.keys = sim, keys
         ctx.device = values[0].device

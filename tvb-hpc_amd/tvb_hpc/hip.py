@@ -103,6 +103,7 @@ _SIGS = {
     'tvbx_sim_set_exec': [_P, _I, _I, _I],
     'tvbx_sim_set_coupling': [_P, _I],
     'tvbx_sim_set_pacing': [_P, _I],
+    'tvbx_sim_set_checkpoint_reuse': [_P, _I],
     'tvbx_sim_checkpoint_run': [_P, _LL, _I],
     'tvbx_sim_adjoint': [_P, _P, _I, _LL, _LL, _I, _P, _I, _P, _I],
     'tvbx_sim_get_outputs': [_P, _P, _P, _P],
@@ -671,6 +672,14 @@ class Simulation:
         per step of device memory): the adjoint then needs no replay."""
         check(lib().tvbx_sim_checkpoint_run(self.h, int(n_steps),
                                             int(window)))
+
+    def set_checkpoint_reuse(self, keep):
+        """keep=True (the default): ``adjoint`` restores each window's
+        checkpoint by a copy and leaves them for another adjoint of the same
+        run.  keep=False: it replays each window inside its checkpoint's
+        buffers (no copies) and consumes them, so a second ``adjoint`` needs
+        a new ``checkpoint_run``.  The gradients are the same."""
+        check(lib().tvbx_sim_set_checkpoint_reuse(self.h, int(bool(keep))))
 
     def adjoint(self, dsamples, sample0=0, n_samples=None):
         """Gradients of a loss from dL/dsample of the samples produced by
