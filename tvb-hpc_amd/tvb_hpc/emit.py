@@ -544,6 +544,7 @@ class KernelSpec:
                 % (type(self.model).__name__, type(self.cfun).__name__,
                    type(self.scheme).__name__))
         src = (head + _read('tvbx_args.h') + '\n' + _read('tvbx_philox.h')
+               + '\n' + _read('tvbx_math.h')
                + '\n' + self.emit_model() + '\n'
                + _read('step_template.hip'))
         if self.adjoint:

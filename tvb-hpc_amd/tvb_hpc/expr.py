@@ -452,16 +452,20 @@ def _d(e, n):
 # ---------------------------------------------------------------------------
 # C / HIP emission
 
-_C_FUN32 = {'sin': 'sinf', 'cos': 'cosf', 'tan': 'tanf', 'exp': 'expf',
+# sin and cos: tvbx_math.h (a bounded-argument path before the library's)
+_C_FUN32 = {'sin': 'tvbx_sinf', 'cos': 'tvbx_cosf', 'tan': 'tanf',
+            'exp': 'expf',
             'expm1': 'expm1f',
             'log': 'logf', 'sqrt': 'sqrtf', 'tanh': 'tanhf', 'abs': 'fabsf',
             'fabs': 'fabsf', 'pow': 'powf', 'sinh': 'sinhf', 'cosh': 'coshf',
             'atan': 'atanf', 'asin': 'asinf', 'acos': 'acosf',
-            'sinf': 'sinf', 'cosf': 'cosf', 'expf': 'expf', 'logf': 'logf',
+            'sinf': 'tvbx_sinf', 'cosf': 'tvbx_cosf', 'expf': 'expf',
+            'logf': 'logf',
             'sqrtf': 'sqrtf'}
 _C_FUN64 = {k: v.rstrip('f') if v.endswith('f') and v != 'fabsf' else v
             for k, v in _C_FUN32.items()}
-_C_FUN64.update({'abs': 'fabs', 'fabs': 'fabs', 'sinf': 'sin', 'cosf': 'cos',
+_C_FUN64.update({'sin': 'sin', 'cos': 'cos',
+                 'abs': 'fabs', 'fabs': 'fabs', 'sinf': 'sin', 'cosf': 'cos',
                  'expf': 'exp', 'logf': 'log', 'sqrtf': 'sqrt'})
 
 
