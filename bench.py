@@ -112,10 +112,22 @@ def device_copy_rate(mib=256, reps=20):
     e1.record()
     e1.synchronize()
     ms = e0.elapsed_time(e1) / reps
-    del x, y
+    # a read-only stream: a sum over 4x the bytes
+    z = torch.ones(4 * n, device='cuda:0')
+    torch.sum(z)
+    e0.record()
+    for _ in range(reps):
+        torch.sum(z)
+    e1.record()
+    e1.synchronize()
+    ms_r = e0.elapsed_time(e1) / reps
+    del x, y, z
     return {'GBs': 2 * n * 4 / (ms * 1e-3) / 1e9, 'us_per_copy': ms * 1e3,
             'what': 'torch.mul(x, 1.0, out=y) over %d MiB (read + write), '
-                    'mean of %d calls' % (mib, reps)}
+                    'mean of %d calls' % (mib, reps),
+            'read_GBs': 4 * n * 4 / (ms_r * 1e-3) / 1e9,
+            'read_what': 'torch.sum over %d MiB (read only), mean of %d '
+                         'calls' % (4 * mib, reps)}
 
 
 def distinct_sources(csr):
@@ -765,6 +777,8 @@ def main():
                     rf['traffic'] / (rf['launch_ms'] * 1e-3) / 1e9
                 cp['step_kernel_vs_copy'] = \
                     cp['step_kernel_traffic_GBs'] / cp['GBs']
+                cp['step_kernel_vs_read'] = \
+                    cp['step_kernel_traffic_GBs'] / cp['read_GBs']
             rf['device_copy'] = cp
         out['value'] = main_r['value']
         out['ms_per_step'] = main_r['ms_per_step']
