@@ -105,10 +105,31 @@ tvbx_adjoint(const tvbx_adj_args A) {
 #endif
   }
 
+  // step n = step0 - k descends: its slot t = n mod H and its sample
+  // n div T (T = tavg_period) step back by one at a time, with no 64-bit
+  // division inside the loop
+  int t_k = (int)(a.step0 % H);
+  long long mq = 0;
+  int mr = 0;
+  if (a.tavg_period > 0) {
+    mq = a.step0 / a.tavg_period;
+    mr = (int)(a.step0 - mq * a.tavg_period);
+  }
   for (int k = 0; k < a.n_steps; ++k) {
+    if (k > 0) {
+      t_k = t_k == 0 ? H - 1 : t_k - 1;
+      if (a.tavg_period > 0) {
+        if (mr == 0) {
+          mr = a.tavg_period - 1;
+          --mq;
+        } else {
+          --mr;
+        }
+      }
+    }
     if (one_node) tvbx_progress_prio(k, a.n_steps);
     const long long n = a.step0 - k;
-    const int t = (int)(n % H);
+    const int t = t_k;
     const size_t w = (size_t)(n - a.rec_step0);
     for (int j0 = i_begin + wave; j0 < i_end; j0 += nwave) {
       const int j = __builtin_amdgcn_readfirstlane(j0);
@@ -252,7 +273,7 @@ tvbx_adjoint(const tvbx_adj_args A) {
             tvbx_dpost_dsum(lp[q], a) * tvbx_dpre(y[0], lp[q], a) * accm[q];
         float fj = 0.0f;
         if (A.forcing != nullptr && a.tavg_period > 0) {
-          const long long m = n / a.tavg_period - A.sample0;
+          const long long m = mq - A.sample0;  // n div T - sample0
           if (m >= 0 && m < A.n_forcing)
             fj = A.forcing[((size_t)m * N + j) * a.pitch + p] * rK;
         }
