@@ -93,41 +93,17 @@ def traffic_from_profile(kernel, workload, launch):
     return float(m.group(1))
 
 
-def device_copy_rate(mib=256, reps=20):
-    """HBM rate of a plain streaming copy on this device (torch's
-    elementwise kernel, `mib` MiB read and `mib` MiB written per call), as
-    the practical ceiling the step kernels' measured traffic compares with.
-    Timed with events on torch's current stream, where the copies run."""
-    import torch
-    n = mib * (1 << 20) // 4
-    x = torch.ones(n, device='cuda:0')
-    y = torch.empty_like(x)
-    for _ in range(3):
-        torch.mul(x, 1.0, out=y)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        torch.mul(x, 1.0, out=y)
-    e1.record()
-    e1.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    # a read-only stream: a sum over 4x the bytes
-    z = torch.ones(4 * n, device='cuda:0')
-    torch.sum(z)
-    e0.record()
-    for _ in range(reps):
-        torch.sum(z)
-    e1.record()
-    e1.synchronize()
-    ms_r = e0.elapsed_time(e1) / reps
-    del x, y, z
-    return {'GBs': 2 * n * 4 / (ms * 1e-3) / 1e9, 'us_per_copy': ms * 1e3,
-            'what': 'torch.mul(x, 1.0, out=y) over %d MiB (read + write), '
-                    'mean of %d calls' % (mib, reps),
-            'read_GBs': 4 * n * 4 / (ms_r * 1e-3) / 1e9,
-            'read_what': 'torch.sum over %d MiB (read only), mean of %d '
-                         'calls' % (4 * mib, reps)}
+def device_copy_rate(mib=1024):
+    """The device's practical streaming rates, the ceiling the step
+    kernels' measured traffic compares with: the best of a grid-stride read
+    and a grid-stride copy of `mib` MiB (libtvbx's tvbx_stream_rates; the
+    copy counts its reads and writes)."""
+    from tvb_hpc.hip import Context
+    read, copy = Context.default().stream_rates(mib)
+    return {'read_GBs': read, 'GBs': copy,
+            'what': 'tvbx_stream_rates over %d MiB: grid-stride read of '
+                    '16-byte words, and grid-stride copy (read + write), '
+                    'best of 3 grid sizes x 10 calls' % mib}
 
 
 def distinct_sources(csr):

@@ -323,6 +323,16 @@ int tvbx_corr(tvbx_ctx *ctx, const float *samples, long long n, int N, int P,
 int tvbx_corr_noalloc(tvbx_ctx *ctx, const float *samples, long long n,
                       int N, int P, float *corr);
 
+/* ---- device stream rates ----------------------------------------------------
+ * No counterpart in the reference: a diagnostic for measurements.  Times a
+ * grid-stride read and a grid-stride copy of `bytes` (a multiple of 16, at
+ * least 1 MiB) on the context's stream at a few grid sizes and returns the
+ * best rates in GB/s (the copy counts its reads and writes): the practical
+ * HBM ceiling that a kernel's measured traffic compares with.  Allocates and
+ * frees two buffers of `bytes`. */
+int tvbx_stream_rates(tvbx_ctx *ctx, size_t bytes, double *read_gbs,
+                      double *copy_gbs);
+
 /* ---- FC-fit loss ------------------------------------------------------------
  * No counterpart in the reference, which has no loss and no gradient (its
  * README.md:3-4 names Bayesian inversion as the use): the FC-fit loss of the

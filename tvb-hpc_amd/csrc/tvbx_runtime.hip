@@ -2434,6 +2434,102 @@ extern "C" int tvbx_transpose(tvbx_ctx *c, const float *in, long long rows,
   return TVBX_OK;
 }
 
+// --------------------------------------------------------- stream rates
+// The device's practical streaming rates, the ceiling the step kernels'
+// measured HBM traffic is compared with: a grid-stride read of 16-byte
+// words (their sum kept live by a store that never happens) and a
+// grid-stride copy.
+__global__ void __launch_bounds__(256)
+k_stream_read(const float4 *__restrict__ in, size_t n,
+              float *__restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (; i < n; i += stride) {
+    const float4 v = in[i];
+    acc.x += v.x;
+    acc.y += v.y;
+    acc.z += v.z;
+    acc.w += v.w;
+  }
+  if (acc.x == -1.0f) out[0] = acc.y + acc.z + acc.w;  // inputs are zeros
+}
+
+__global__ void __launch_bounds__(256)
+k_stream_copy(const float4 *__restrict__ in, float4 *__restrict__ out,
+              size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = in[i];
+}
+
+extern "C" int tvbx_stream_rates(tvbx_ctx *c, size_t bytes, double *read_gbs,
+                                 double *copy_gbs) {
+  CHECK(c && read_gbs && copy_gbs, "null argument");
+  CHECK(bytes >= ((size_t)1 << 20) && bytes % 16 == 0,
+        "bytes must be a multiple of 16 and at least 1 MiB");
+  HIPCHK(hipSetDevice(c->device));
+  DevBuf a, b, o;
+  int rc = dalloc(a, bytes);
+  if (rc == TVBX_OK) rc = dalloc(b, bytes);
+  if (rc == TVBX_OK) rc = dalloc(o, 256);
+  if (rc != TVBX_OK) {
+    dfree(a);
+    dfree(b);
+    dfree(o);
+    return rc;
+  }
+  hipStream_t st = c->stream;
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(k_fill_f32, dim3(grid1d(bytes / 4, 256)), dim3(256), 0,
+                     st, (float *)a.p, bytes / 4, 0.0f);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
+  double best_r = 0.0, best_c = 0.0;
+  const unsigned grids[] = {1024u, 4096u, 65536u};
+  const int reps = 10;
+  for (unsigned g : grids) {
+    if (he != hipSuccess) break;
+    for (int w = 0; w < 2; ++w)
+      hipLaunchKernelGGL(k_stream_read, dim3(g), dim3(256), 0, st,
+                         (const float4 *)a.p, n, (float *)o.p);
+    hipEventRecord(e0, st);
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_stream_read, dim3(g), dim3(256), 0, st,
+                         (const float4 *)a.p, n, (float *)o.p);
+    hipEventRecord(e1, st);
+    he = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+    if (he == hipSuccess && ms > 0.0f)
+      best_r = std::max(best_r, (double)bytes * reps / (ms * 1e-3) / 1e9);
+    for (int w = 0; w < 2; ++w)
+      hipLaunchKernelGGL(k_stream_copy, dim3(g), dim3(256), 0, st,
+                         (const float4 *)a.p, (float4 *)b.p, n);
+    hipEventRecord(e0, st);
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_stream_copy, dim3(g), dim3(256), 0, st,
+                         (const float4 *)a.p, (float4 *)b.p, n);
+    hipEventRecord(e1, st);
+    if (he == hipSuccess) he = hipEventSynchronize(e1);
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+    if (he == hipSuccess && ms > 0.0f)
+      best_c = std::max(best_c, 2.0 * bytes * reps / (ms * 1e-3) / 1e9);
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipStreamSynchronize(st);
+  dfree(a);
+  dfree(b);
+  dfree(o);
+  if (he != hipSuccess)
+    return fail(TVBX_E_HIP, "stream rates: %s", hipGetErrorString(he));
+  *read_gbs = best_r;
+  *copy_gbs = best_c;
+  return TVBX_OK;
+}
+
 // ------------------------------------------------------------ region maps
 // Surface vertices <-> regions (the reference's RMapToAvg / RMapFromAvg,
 // tvb_hpc/network.py:174-202).  Arrays are [rows][nvar][n_inst] with the

@@ -119,6 +119,7 @@ _SIGS = {
     'tvbx_corr_noalloc': [_P, _P, _LL, _I, _I, _P],
     'tvbx_fc_loss': [_P, _P, _LL, _I, _I, _P, _I, _P, _P, _P],
     'tvbx_transpose': [_P, _P, _LL, _LL, _P, _I, _P],
+    'tvbx_stream_rates': [_P, _SZ, _P, _P],
     'tvbx_rmap_create': [_P, _P, _I, _I, _P],
     'tvbx_rmap_destroy': [_P],
     'tvbx_rmap_to_avg': [_P, _I, _I, _P, _P],
@@ -295,6 +296,15 @@ class Context:
                                  ctypes.c_void_p(loss_ptr),
                                  ctypes.c_void_p(s_ptr),
                                  ctypes.c_void_p(mean_ptr)))
+
+    def stream_rates(self, mib=1024):
+        """(read GB/s, copy GB/s): the best of a grid-stride read and a
+        grid-stride copy of ``mib`` MiB on this device (the copy counts its
+        reads and writes)."""
+        r, c = ctypes.c_double(), ctypes.c_double()
+        check(lib().tvbx_stream_rates(self.h, int(mib) << 20, ctypes.byref(r),
+                                      ctypes.byref(c)))
+        return r.value, c.value
 
     def transpose(self, in_ptr, rows, cols, out_ptr, sub_ptr=None,
                   sub_rows=1):
