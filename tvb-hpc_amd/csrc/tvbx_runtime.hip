@@ -2987,6 +2987,9 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
     if (overlap) HIPCHK(hipEventRecord(ev_rep(kw), sb));
     return TVBX_OK;
   };
+  // from here on the checkpoints may be overwritten: a call that fails
+  // part-way leaves them marked consumed as well
+  if (consume) s->ck_consumed = true;
   if (replay) {
     rc = issue_replay(K - 1);
     if (rc != TVBX_OK) return rc;
@@ -3034,7 +3037,6 @@ extern "C" int tvbx_sim_adjoint(tvbx_sim *s, const float *dsamples,
     HIPCHK(hipMemcpyAsync(s->ring.p, s->save_ring.p, s->ring.bytes,
                           hipMemcpyDeviceToDevice, s->ctx->stream));
   }
-  s->ck_consumed = consume;
   s->step = step_end;
   s->clock_dirty = true;
   // outputs: coupling gradients summed over nodes [n_gc][P]; model
