@@ -32,8 +32,8 @@ fc_err_vs_fp64
 full_run     wall-clock rate of those full-length runs
 grad         gradient evaluations/s of the FC-fit inversion workload (MPR
              N=84, P=4096, 20000 steps, checkpoints every 1000 steps replayed
-             in the backward pass): forward + FC loss + adjoint reverse sweep
-             for all instances
+             in the backward pass): forward + FC loss on the device + adjoint
+             reverse sweep for all instances
 roofline     the step kernel against HBM bandwidth: algorithmic bytes per
              launch (1, 2 or 4 B per edge gather + per node write, per instance
              and step, times the steps of the launch) / mean launch duration
@@ -41,7 +41,10 @@ roofline     the step kernel against HBM bandwidth: algorithmic bytes per
              each distinct (source, delay) history value once per step
              instead of once per edge; `traffic` is the measured HBM bytes per
              launch from the committed rocprofv3 PMC summary for this kernel,
-             workload and launch shape
+             workload and launch shape; `device_copy` the device's own
+             streaming rates (grid-stride read and copy over 1 GiB,
+             tvbx_stream_rates), the practical ceiling that traffic rate
+             compares with
 cpu_baseline the float32 CPU oracle (the reference's loop structure: per
              instance, sequential steps, CSR row reduction, OpenMP over
              instances), built on the host with -O3 -march=native with and
