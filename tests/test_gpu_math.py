@@ -1,8 +1,9 @@
-"""The emitted sine and cosine (tvbx_math.h: a bounded-argument reduction
-and polynomials, the library's functions beyond |x| = 100) against float64
-numpy, through a model whose drift is (0, sin x, cos x): one deterministic
-Euler step of dt = 1 from s = c = 0 leaves s = sin(x0) and c = cos(x0)
-exactly as the device computed them."""
+"""The emitted sine, cosine and division (tvbx_math.h: a bounded-argument
+reduction and polynomials, the library's functions beyond |x| = 100; a
+reciprocal corrected once by its residual) against float64 numpy, through
+models whose drift is (0, sin x, cos x) or (0, 0, x / y): one deterministic
+Euler step of dt = 1 from zeros leaves the values exactly as the device
+computed them."""
 
 import numpy as np
 import pytest
@@ -68,3 +69,56 @@ def test_emitted_sin_cos_within_two_ulp():
     assert es[~small].max() <= 2.0 and ec[~small].max() <= 2.0
     # signs of zero and exact values at zero
     assert out[1][0, 0] == 0.0 and out[2][0, 0] == 1.0
+
+
+class _Div(model.BaseModel):
+    "x, y constant; q' = x / y."
+    state = 'x y q'
+    input = 'i'
+    param = 'a'
+    drift = '0', '0', 'x / y'
+    diffs = 0, 0, 0
+    obsrv = 'x',
+    const = {'a': 0.0}
+    limit = (-1, 1), (-1, 1), (-1, 1)
+
+
+def test_emitted_division_within_one_ulp():
+    N, P = 16, 4096
+    csr = small_csr(N=N, density=0.3, seed=3, speeds=(2.0, ), dt=0.1)
+    m = _Div()
+    c = coupling.Linear(m)
+    c.param['a'] = 0.0
+    rng = np.random.default_rng(11)
+    n = N * P
+    # magnitudes over many binades, both signs, and quotients near 1
+    x = (rng.choice([-1.0, 1.0], n) *
+         np.exp2(rng.uniform(-60, 60, n))).astype(np.float32)
+    y = (rng.choice([-1.0, 1.0], n) *
+         np.exp2(rng.uniform(-60, 60, n))).astype(np.float32)
+    y[: n // 8] = (x[: n // 8] * rng.uniform(0.99, 1.01, n // 8)).astype(
+        np.float32)
+    # special operands: infinite and zero divisors and dividends
+    sx = np.array([1.0, -3.0, 0.0, 5.0, np.inf, 0.0, 2.0], np.float32)
+    sy = np.array([np.inf, -np.inf, 7.0, 0.0, 3.0, 0.0, 1e-30], np.float32)
+    x[:sx.size], y[:sy.size] = sx, sy
+    x, y = x.reshape(N, P), y.reshape(N, P)
+    sim = Simulation(m, c, scheme.EulerStep(1.0), csr, P, waves=4,
+                     node_block=4)
+    sim.set_state(np.stack([x, y, np.zeros_like(x)]).astype(np.float32))
+    sim.init_history('zeros')
+    sim.run(1)
+    out = sim.get_state()
+    with np.errstate(divide='ignore', invalid='ignore'):
+        ref = x.astype(np.float64) / y.astype(np.float64)
+    q = out[2].ravel()
+    # x / +-inf = +-0, 0 / 7 = 0, 5 / 0 = inf, inf / 3 = inf, 0 / 0 = NaN
+    assert q[0] == 0.0 and q[1] == 0.0 and q[2] == 0.0
+    assert q[3] == np.inf and q[4] == np.inf and np.isnan(q[5])
+    ok = np.isfinite(ref)
+    ok.ravel()[:sx.size] = False
+    e = _ulps(out[2][ok], ref[ok])
+    print('division: max %.3f ulp, correctly rounded %.4f' % (
+        e.max(), np.mean(out[2][ok] == ref[ok].astype(np.float32))))
+    assert e.max() <= 1.0
+

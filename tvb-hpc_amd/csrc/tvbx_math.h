@@ -48,4 +48,18 @@ __device__ __forceinline__ float tvbx_cosf(float x) {
   return tvbx_sincos_reduced(x, 1);
 }
 
+// a / b within one ulp (faithful rounding): the reciprocal's estimate
+// (v_rcp_f32, about one ulp) times a, corrected once by the residual,
+// q = a r, q + r (a - q b) -- four instructions against the ten or so of
+// the correctly rounded division.  The first quotient is returned as it is
+// when it or b is not finite (b = 0, b = +-inf -- where a / b is +-0 but the
+// residual would be 0 * inf -- and NaN operands).
+__device__ __forceinline__ float tvbx_divf(float a, float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  const float q = a * r;
+  const float e = fmaf(-q, b, a);
+  return (__builtin_isfinite(q) && __builtin_isfinite(b)) ? fmaf(e, r, q)
+                                                            : q;
+}
+
 #endif

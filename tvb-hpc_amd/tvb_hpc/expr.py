@@ -512,6 +512,10 @@ def to_c(e: Expr, names: Dict[str, str] = None, ctype='float') -> str:
                     base = go(x.a)
                     return '(' + '*'.join([base] * k) + ')'
                 return '%s(%s, %s)' % (funs['pow'], go(x.a), go(x.b))
+            if x.op == '/' and ctype == 'float':
+                # within one ulp, in a third of the correctly rounded
+                # division's instructions (tvbx_math.h)
+                return 'tvbx_divf(%s, %s)' % (go(x.a), go(x.b))
             return '(%s %s %s)' % (go(x.a), x.op, go(x.b))
         if isinstance(x, Call):
             fn = funs.get(x.fn)
