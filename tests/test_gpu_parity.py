@@ -134,6 +134,30 @@ def test_reference_order_parity(model, cfun, dt, cparams, launch):
     assert scaled_err(sim.get_ring(), ref.get_ring()) < TOL
 
 
+@pytest.mark.parametrize('launch', LAUNCHES)
+def test_kuramoto_far_phases_take_the_checked_path(launch):
+    """Phases far outside [0, 2 pi) (the model wraps them by one period
+    per step) put the coupling's sine arguments beyond the reduced range in
+    some lanes and not others: those gather groups are redone through the
+    library's sine, and the trajectory still follows the oracle."""
+    sim, ref, noise = _pair('Kuramoto', 'Kuramoto', 0.1, {'a': 1.5},
+                            launch=launch, steps=100)
+    rng = np.random.default_rng(4)
+    x0 = sim.get_state()
+    far = rng.random(x0.shape) < 0.3
+    x0 = np.where(far, x0 + 2 * np.pi * rng.integers(20, 200, x0.shape),
+                  x0).astype(np.float32)
+    sim.set_state(x0)
+    ref.set_state(x0)
+    sim.init_history('current')
+    ref.init_history('current')
+    sim.run(100, noise=noise)
+    ref.run(100, noise=noise)
+    xg, xr = sim.get_state(), ref.get_state()
+    assert np.isfinite(xg).all()
+    assert state_err('Kuramoto', xg, xr) < TOL
+
+
 @pytest.mark.parametrize('model,cfun,dt,cparams', CASES[:4])
 def test_trajectory_parity_deterministic(model, cfun, dt, cparams):
     """Without noise; 500 steps, because the delayed Kuramoto network is

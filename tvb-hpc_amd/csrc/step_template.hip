@@ -486,6 +486,61 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
         LD(sx[u]);                                                          \
     __builtin_amdgcn_sched_barrier(0);                                      \
   }
+#if TVBX_PRE_NB && TVBX_RING_T != 3
+// pre() with sines or cosines: a group's terms through the branch-free
+// tvbx_pre_nb (the sines interleave), then one check -- a lane whose sum
+// turned NaN in this group (an argument outside the reduced range, or a
+// non-finite value) redoes the group through tvbx_pre.  In range both give
+// the same values, so the sums are those of tvbx_pre alone.
+template <typename LP>
+__device__ __forceinline__ void tvbx_accum_nb(float (&acc)[TVBX_IPL],
+                                              float w, const tvbx_raw &v,
+                                              const float (&post)[TVBX_IPL],
+                                              const LP (&lp)[TVBX_IPL],
+                                              const tvbx_step_args &a,
+                                              int kk) {
+  float f[TVBX_IPL];
+  tvbx_unpack(v, f);
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q)
+    acc[q] = fmaf(w, tvbx_pre_nb(kk, f[q], post[q], lp[q], a), acc[q]);
+}
+__device__ __forceinline__ bool tvbx_group_redo(const float (&acc0)[TVBX_IPL],
+                                                const float (&acc)[TVBX_IPL]) {
+  bool r = false;
+#pragma unroll
+  for (int q = 0; q < TVBX_IPL; ++q)
+    r = r || (__builtin_isnan(acc[q]) && !__builtin_isnan(acc0[q]));
+  return r;
+}
+#define TVBX_GROUP_SUM(WEXPR)                                                 \
+  {                                                                         \
+    float acc0_[TVBX_IPL];                                                  \
+    _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc0_[q] = acc[q]; \
+    _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {               \
+      WEXPR;                                                                \
+      tvbx_accum_nb(acc, w_, V_[u], post, lp, a, kk);                       \
+    }                                                                       \
+    if (tvbx_group_redo(acc0_, acc)) {                                      \
+      _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) acc[q] = acc0_[q]; \
+      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {             \
+        WEXPR;                                                              \
+        tvbx_accum(acc, w_, V_[u], post, lp, a, kk);                        \
+      }                                                                     \
+    }                                                                       \
+  }
+#define TVBX_GROUP_USE(V, G0, WW)                                             \
+  {                                                                         \
+    auto &V_ = V;                                                           \
+    TVBX_GROUP_SUM(const float w_ = WW((G0) + u))                           \
+  }
+#define TVBX_GROUP_USE_LAST(V, G0, WW, NE)                                    \
+  {                                                                         \
+    auto &V_ = V;                                                           \
+    TVBX_GROUP_SUM(const int e_ = min((G0) + u, (NE) - 1);                  \
+                   const float w_ = (G0) + u < (NE) ? WW(e_) : 0.0f)        \
+  }
+#else
 #define TVBX_GROUP_USE(V, G0, WW)                                             \
   _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {                 \
     tvbx_accum(acc, WW((G0) + u), V[u], post, lp, a, kk);                   \
@@ -496,6 +551,7 @@ __device__ __forceinline__ tvbx_raw tvbx_rowload(const char *ring,
     const float w_ = (G0) + u < (NE) ? WW(e_) : 0.0f;                       \
     tvbx_accum(acc, w_, V[u], post, lp, a, kk);                             \
   }
+#endif
 #if TVBX_PIPE
 // Pipelined: the next group's loads are issued before the current group's
 // values are consumed, so up to 2 * TVBX_UNROLL gathers are in flight; two

@@ -48,6 +48,20 @@ __device__ __forceinline__ float tvbx_cosf(float x) {
   return tvbx_sincos_reduced(x, 1);
 }
 
+// The same without the branch, for a group of coupling terms whose range
+// is checked once (the step kernels' gather groups): NaN outside |x| <= 100,
+// which makes the group's sum NaN and has it redone through tvbx_sinf /
+// tvbx_cosf.  With no branch between them, the sines of a group interleave.
+__device__ __forceinline__ float tvbx_sinf_nb(float x) {
+  const float v = tvbx_sincos_reduced(x, 0);
+  return fabsf(x) <= 100.0f ? v : __builtin_nanf("");
+}
+
+__device__ __forceinline__ float tvbx_cosf_nb(float x) {
+  const float v = tvbx_sincos_reduced(x, 1);
+  return fabsf(x) <= 100.0f ? v : __builtin_nanf("");
+}
+
 // a / b within one ulp (faithful rounding): the reciprocal's estimate
 // (v_rcp_f32, about one ulp) times a, corrected once by the residual,
 // q = a r, q + r (a - q b) -- four instructions against the ten or so of

@@ -298,7 +298,7 @@ class KernelSpec:
         lines.append('}')
         # coupling: pre / post
         cn = self._cfun_names()
-        pre_c, post_c, ringv = [], [], []
+        pre_c, post_c, ringv, pre_nb = [], [], [], []
         for k in range(C):
             pre = self.cfun.pre_expr(k)
             self._check_names(pre, dict(cn, pre_syn=1, post_syn=1),
@@ -319,6 +319,7 @@ class KernelSpec:
                                               post_syn='0.0f')))
             else:
                 pre_c.append(pre_src)
+                pre_nb.append(E.to_c(pre, names, unchecked_trig=True))
                 ringv.append('y')
         for r in range(C, R):
             ringv.append('y')
@@ -329,6 +330,17 @@ class KernelSpec:
             lines.append('  if (kk == %d) return %s;' % (k, pre_c[k]))
         lines.append('  return 0.0f;')
         lines.append('}')
+        # with sines or cosines in pre: a branch-free version whose gather
+        # groups are checked once (NaN outside the reduced range)
+        nb = bool(pre_nb) and pre_nb != pre_c[:len(pre_nb)]
+        lines.append('#define TVBX_PRE_NB %d' % int(nb))
+        if nb:
+            lines.append('__device__ __forceinline__ float tvbx_pre_nb(%s) {'
+                         % sig)
+            for k in range(C):
+                lines.append('  if (kk == %d) return %s;' % (k, pre_nb[k]))
+            lines.append('  return 0.0f;')
+            lines.append('}')
         lines.append('__device__ __forceinline__ float tvbx_post(int kk, '
                      'float sum, const tvbx_lane_params &lp, '
                      'const tvbx_step_args &a) {')

@@ -485,14 +485,20 @@ def c_literal(value, ctype='float') -> str:
     return '(%s)' % v.hex() if v < 0 else v.hex()
 
 
-def to_c(e: Expr, names: Dict[str, str] = None, ctype='float') -> str:
+def to_c(e: Expr, names: Dict[str, str] = None, ctype='float',
+         unchecked_trig=False) -> str:
     """Emit ``e`` as a C expression.
 
     ``names`` maps variable names to C expressions (default: identity).
     Integer powers 2..4 become multiplication chains; other powers call
-    ``pow``.  Comparisons yield ``(T)(a < b)``."""
+    ``pow``.  Comparisons yield ``(T)(a < b)``.  ``unchecked_trig`` (float
+    only) calls the branch-free sine and cosine, NaN outside their reduced
+    range, for code that checks a group of results at once."""
     names = names or {}
     funs = _C_FUN32 if ctype == 'float' else _C_FUN64
+    if unchecked_trig and ctype == 'float':
+        funs = dict(funs, sin='tvbx_sinf_nb', sinf='tvbx_sinf_nb',
+                    cos='tvbx_cosf_nb', cosf='tvbx_cosf_nb')
 
     def go(x):
         if isinstance(x, Num):
