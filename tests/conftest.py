@@ -14,7 +14,7 @@ GOLDEN = os.path.join(ROOT, 'tests', 'golden')
 
 def pytest_configure(config):
     config.addinivalue_line(
-        'markers', 'gpu: needs an MI355X (run with -m gpu on the GPU box)')
+        'markers', 'gpu: needs an MI355X device (select with -m gpu)')
     config.addinivalue_line('markers', 'slow: longer CPU test')
 
 
