@@ -198,6 +198,24 @@ def test_adjoint_consuming_checkpoints_gives_the_same_gradients():
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
+def test_second_backward_of_one_integrate_raises():
+    """integrate's backward replays inside the checkpoints: a second
+    backward of the same forward fails loudly instead of reading them."""
+    from tvb_hpc.hip import TvbxError
+    csr, m, c, sc, G, eta, x0 = _setup()
+    s_t = _oracle_samples(csr, np.full(P, 0.5), eta, x0)
+    fct = AG.fc(torch.from_numpy(s_t))[0].numpy()
+    sim = _device_sim(csr, m, c, sc, G, eta)
+    Gt = torch.tensor(G, device='cuda', requires_grad=True)
+    et = torch.tensor(eta, device='cuda', requires_grad=True)
+    smp = AG.integrate(sim, {'c:a': Gt, 'm:eta': et}, x0, T, WIN, seed=SEED)
+    L = AG.fc_loss(smp, fct).sum()
+    L.backward(retain_graph=True)
+    assert torch.isfinite(Gt.grad).all()
+    with pytest.raises(TvbxError, match='consumed'):
+        L.backward()
+
+
 def test_adjoint_with_temporal_blocking():
     """Forward replay in blocked launches and reverse launches of several
     steps give the same samples and gradients, bit for bit, as one-step
