@@ -219,14 +219,22 @@ def test_second_backward_of_one_integrate_raises():
 def test_adjoint_with_temporal_blocking():
     """Forward replay in blocked launches and reverse launches of several
     steps give the same samples and gradients, bit for bit, as one-step
-    launches."""
+    launches; so do reverse sweeps that gather their transposed rows in
+    groups of 4 or 32 (the same additions in the same order), with one
+    node per wave (lambda kept in registers) or four."""
     csr, m, c, sc, G, eta, x0 = _setup()
     L = min(int(csr.delays.min()) + 1, 8)
     assert L > 2
     dS = None
     out = []
     for launch in (dict(waves=4, node_block=4),
-                   dict(waves=4, node_block=4, steps_per_launch=L)):
+                   dict(waves=4, node_block=4, steps_per_launch=L),
+                   dict(waves=4, node_block=4, steps_per_launch=L,
+                        adj_unroll=32),
+                   dict(waves=1, node_block=1, steps_per_launch=L,
+                        adj_unroll=4),
+                   dict(waves=1, node_block=1, steps_per_launch=L,
+                        adj_unroll=32)):
         sim = Simulation(m, c, sc, csr, P, params={'c:a': G, 'm:eta': eta},
                          kinds={'c:a': 'inst', 'm:eta': 'ni'}, tavg_period=K,
                          sample_slots=T // K, adjoint=True, **launch)
@@ -239,9 +247,26 @@ def test_adjoint_with_temporal_blocking():
             dS = np.random.default_rng(1).standard_normal(
                 smp.shape).astype(np.float32)
         out.append((smp, sim.adjoint(dS, first)))
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    for k in out[0][1]:
-        np.testing.assert_array_equal(out[0][1][k], out[1][1][k])
+    for smp, g in out[1:]:
+        np.testing.assert_array_equal(out[0][0], smp)
+        for k in out[0][1]:
+            np.testing.assert_array_equal(out[0][1][k], g[k])
+
+
+def test_adjoint_unroll_32_refuses_wide_workgroups():
+    """The adjoint with 32 gathers in flight is built for at most 8 waves
+    per workgroup: a 16-wave launch is refused before anything runs."""
+    from tvb_hpc.hip import TvbxError
+    csr, m, c, sc, G, eta, x0 = _setup()
+    sim = Simulation(m, c, sc, csr, P, params={'c:a': G, 'm:eta': eta},
+                     kinds={'c:a': 'inst', 'm:eta': 'ni'}, tavg_period=K,
+                     sample_slots=T // K, adjoint=True, waves=16,
+                     adj_unroll=32)
+    sim.set_state(x0)
+    sim.init_history('zeros')
+    with pytest.raises(TvbxError, match='waves per workgroup'):
+        sim.checkpoint_run(T, WIN)
+    assert sim.step == 0
 
 
 def test_autograd_device_tensors_match_host_tensors():

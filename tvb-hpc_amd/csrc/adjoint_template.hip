@@ -49,7 +49,15 @@ __device__ __forceinline__ void tvbx_bloadf(__amdgpu_buffer_rsrc_t r,
 #endif
 }
 
-extern "C" __global__ void __launch_bounds__(1024)
+// 32 gathers in flight need more than the 128 registers a 1024-thread
+// workgroup leaves each lane: such modules take at most 8 waves per
+// workgroup (the runtime checks the launch against this bound)
+#if TVBX_ADJ_UNROLL > 16
+#define TVBX_ADJ_THREADS 512
+#else
+#define TVBX_ADJ_THREADS 1024
+#endif
+extern "C" __global__ void __launch_bounds__(TVBX_ADJ_THREADS)
 tvbx_adjoint(const tvbx_adj_args A) {
   const tvbx_step_args &a = A.f;
   const int lane = threadIdx.x & 63;
@@ -135,6 +143,28 @@ tvbx_adjoint(const tvbx_adj_args A) {
       const int j = __builtin_amdgcn_readfirstlane(j0);
       const size_t jp = (size_t)j * a.pitch + pb;
       if (!one_node) tvbx_load_row_of(ccol, crow, cw, j, crw);
+      // the replay's recordings of step n, lambda (when it is not kept in
+      // registers) and the forcing of n's sample do not depend on the
+      // gathers: their loads are issued first and land while the gathers
+      // are in flight
+      float xs[TVBX_S][TVBX_IPL], ls[TVBX_S][TVBX_IPL], sm[TVBX_IPL],
+          fo[TVBX_IPL];
+#pragma unroll
+      for (int s = 0; s < TVBX_S; ++s) {
+        tvbx_ldf(A.rec_x + (w * TVBX_S + s) * NP + jp, xs[s]);
+        if (!keep) tvbx_ldf(A.lam + s * NP + jp, ls[s]);
+      }
+      tvbx_ldf(A.rec_sum + w * TVBX_C * NP + jp, sm);
+      {
+        const long long m = mq - A.sample0;  // n div T - sample0
+        if (A.forcing != nullptr && a.tavg_period > 0 && m >= 0 &&
+            m < A.n_forcing) {
+          tvbx_ldf(A.forcing + ((size_t)m * N + j) * a.pitch + pb, fo);
+        } else {
+#pragma unroll
+          for (int q = 0; q < TVBX_IPL; ++q) fo[q] = 0.0f;
+        }
+      }
       // mu of the targets of j's outgoing edges at steps n + d: edges in
       // CSC order, indices read back from the row's registers
       float accm[TVBX_IPL];
@@ -152,16 +182,16 @@ tvbx_adjoint(const tvbx_adj_args A) {
 #define TVBX_ADJ_BRANGE(NE, OV, WV)                                           \
   {                                                                           \
     int u0 = 0;                                                               \
-    for (; u0 + TVBX_UNROLL <= (NE); u0 += TVBX_UNROLL) {                     \
-      unsigned so[TVBX_UNROLL];                                               \
-      float v[TVBX_UNROLL][TVBX_IPL];                                         \
-      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) so[u] =         \
+    for (; u0 + TVBX_ADJ_UNROLL <= (NE); u0 += TVBX_ADJ_UNROLL) {             \
+      unsigned so[TVBX_ADJ_UNROLL];                                           \
+      float v[TVBX_ADJ_UNROLL][TVBX_IPL];                                     \
+      _Pragma("unroll") for (int u = 0; u < TVBX_ADJ_UNROLL; ++u) so[u] =     \
           tvbx_lane_u((OV), u0 + u);                                          \
       __builtin_amdgcn_sched_barrier(0);                                      \
-      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u)                 \
+      _Pragma("unroll") for (int u = 0; u < TVBX_ADJ_UNROLL; ++u)             \
           tvbx_bloadf(mrs, lane_off, so[u], v[u]);                            \
       __builtin_amdgcn_sched_barrier(0);                                      \
-      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {               \
+      _Pragma("unroll") for (int u = 0; u < TVBX_ADJ_UNROLL; ++u) {           \
         const float we = tvbx_lane_f((WV), u0 + u);                           \
         _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) accm[q] =        \
             fmaf(we, v[u][q], accm[q]);                                       \
@@ -170,32 +200,32 @@ tvbx_adjoint(const tvbx_adj_args A) {
     if (u0 < (NE)) {                                                          \
       /* the remainder as one group: a slot past the row reads the row's */   \
       /* last edge again, with the weight 0 its lane holds (adds nothing) */  \
-      unsigned so[TVBX_UNROLL];                                               \
-      float v[TVBX_UNROLL][TVBX_IPL];                                         \
-      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) so[u] =         \
+      unsigned so[TVBX_ADJ_UNROLL];                                           \
+      float v[TVBX_ADJ_UNROLL][TVBX_IPL];                                     \
+      _Pragma("unroll") for (int u = 0; u < TVBX_ADJ_UNROLL; ++u) so[u] =     \
           tvbx_lane_u((OV), min(u0 + u, (NE) - 1));                           \
       __builtin_amdgcn_sched_barrier(0);                                      \
-      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u)                 \
+      _Pragma("unroll") for (int u = 0; u < TVBX_ADJ_UNROLL; ++u)             \
           tvbx_bloadf(mrs, lane_off, so[u], v[u]);                            \
       __builtin_amdgcn_sched_barrier(0);                                      \
-      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {               \
+      _Pragma("unroll") for (int u = 0; u < TVBX_ADJ_UNROLL; ++u) {           \
         const float we = tvbx_lane_f((WV), u0 + u);                           \
         _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) accm[q] =        \
             fmaf(we, v[u][q], accm[q]);                                       \
       }                                                                       \
     }                                                                         \
   }
-// TVBX_UNROLL gathers in flight, accumulated in CSC order
+// TVBX_ADJ_UNROLL gathers in flight, accumulated in CSC order
 #define TVBX_ADJ_RANGE(NE, CV, WV)                                            \
   {                                                                           \
     int u0 = 0;                                                               \
-    for (; u0 + TVBX_UNROLL <= (NE); u0 += TVBX_UNROLL) {                     \
-      float v[TVBX_UNROLL][TVBX_IPL];                                         \
-      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {               \
+    for (; u0 + TVBX_ADJ_UNROLL <= (NE); u0 += TVBX_ADJ_UNROLL) {             \
+      float v[TVBX_ADJ_UNROLL][TVBX_IPL];                                     \
+      _Pragma("unroll") for (int u = 0; u < TVBX_ADJ_UNROLL; ++u) {           \
         const unsigned c = tvbx_lane_u((CV), u0 + u);                         \
         tvbx_ldf(TVBX_ADJ_SRC(c), v[u]);                                      \
       }                                                                       \
-      _Pragma("unroll") for (int u = 0; u < TVBX_UNROLL; ++u) {               \
+      _Pragma("unroll") for (int u = 0; u < TVBX_ADJ_UNROLL; ++u) {           \
         const float we = tvbx_lane_f((WV), u0 + u);                           \
         _Pragma("unroll") for (int q = 0; q < TVBX_IPL; ++q) accm[q] =        \
             fmaf(we, v[u][q], accm[q]);                                       \
@@ -241,18 +271,12 @@ tvbx_adjoint(const tvbx_adj_args A) {
 #undef TVBX_ADJ_RANGE
 #undef TVBX_ADJ_BRANGE
 #undef TVBX_ADJ_SRC
-      float xs[TVBX_S][TVBX_IPL], ls[TVBX_S][TVBX_IPL], sm[TVBX_IPL];
+      if (keep) {
 #pragma unroll
-      for (int s = 0; s < TVBX_S; ++s) {
-        tvbx_ldf(A.rec_x + (w * TVBX_S + s) * NP + jp, xs[s]);
-        if (keep) {
+        for (int s = 0; s < TVBX_S; ++s)
 #pragma unroll
           for (int q = 0; q < TVBX_IPL; ++q) ls[s][q] = lr[s][q];
-        } else {
-          tvbx_ldf(A.lam + s * NP + jp, ls[s]);
-        }
       }
-      tvbx_ldf(A.rec_sum + w * TVBX_C * NP + jp, sm);
       float mu_o[TVBX_IPL], lo[TVBX_S][TVBX_IPL];
 #pragma unroll
       for (int q = 0; q < TVBX_IPL; ++q) {
@@ -271,12 +295,7 @@ tvbx_adjoint(const tvbx_adj_args A) {
         tvbx_obsrv(x, y, lp[q], a, j, p);
         const float aj =
             tvbx_dpost_dsum(lp[q], a) * tvbx_dpre(y[0], lp[q], a) * accm[q];
-        float fj = 0.0f;
-        if (A.forcing != nullptr && a.tavg_period > 0) {
-          const long long m = mq - A.sample0;  // n div T - sample0
-          if (m >= 0 && m < A.n_forcing)
-            fj = A.forcing[((size_t)m * N + j) * a.pitch + p] * rK;
-        }
+        const float fj = fo[q] * rK;
         tvbx_dobsrv(0, x, dhc, lp[q], a, j, p);
         tvbx_dobsrv(a.bold_obs, x, dhm, lp[q], a, j, p);
 #pragma unroll

@@ -2732,6 +2732,12 @@ static int adjoint_checks(tvbx_sim *s) {
   CHECK(d.ring_type == 0, "the adjoint needs a float32 history");
   CHECK(!d.bold, "the adjoint does not cover the BOLD stage");
   CHECK(d.n_coupled == 1, "the adjoint needs one coupled observable");
+  int max_bx = 0;
+  HIPCHK(hipFuncGetAttribute(&max_bx, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK,
+                             s->mod->adj));
+  CHECK(64 * d.waves <= max_bx,
+        "the adjoint of this module takes at most %d waves per workgroup "
+        "(adj_unroll 32 takes at most 8)", max_bx / 64);
   return TVBX_OK;
 }
 

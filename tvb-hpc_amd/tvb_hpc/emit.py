@@ -96,12 +96,18 @@ class KernelSpec:
     def __init__(self, model: BaseModel, cfun: BaseCoupling, scheme: EulerStep,
                  kinds=None, sigma_kind=None, ring='coupled', ring_pre=None,
                  order='cuda', ipl=1, ring_dtype='f32', adjoint=False,
-                 unroll=8, min_waves=1, pipe=False, prio=True, emaj=0):
+                 unroll=8, min_waves=1, pipe=False, prio=True, emaj=0,
+                 adj_unroll=0):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.adjoint = bool(adjoint)
         if unroll not in (4, 8, 16, 32):
             raise ValueError('unroll must be 4, 8, 16 or 32')
         self.unroll = int(unroll)
+        # gathers in flight per group of the adjoint's transposed rows
+        # (0: as many as the forward's)
+        if adj_unroll not in (0, 4, 8, 16, 32):
+            raise ValueError('adj_unroll must be 0, 4, 8, 16 or 32')
+        self.adj_unroll = int(adj_unroll)
         # waves per SIMD the blocked kernel's register budget must allow
         if min_waves not in (1, 2, 3, 4, 5, 6, 8):
             raise ValueError('min_waves must be 1 .. 8')
@@ -472,7 +478,9 @@ class KernelSpec:
         cn = self._cfun_names()
         gc, gm = self.grad_params()
         drift = self._drift_inlined()
-        lines = ['#define TVBX_NGC %d' % max(len(gc), 1),
+        lines = ['#define TVBX_ADJ_UNROLL %d' % (self.adj_unroll or
+                                                 self.unroll),
+                 '#define TVBX_NGC %d' % max(len(gc), 1),
                  '#define TVBX_NGM %d' % max(len(gm), 1),
                  '#define TVBX_HAS_NGC %d' % len(gc),
                  '#define TVBX_HAS_NGM %d' % len(gm)]

@@ -429,7 +429,9 @@ class Simulation:
         their model / coupling defaults as literals.
     monitor: tavg_period (steps), bold (bool), fc (bool), sample_slots.
     launch: ipl, waves, node_block, steps_per_launch, unroll, min_waves,
-        pipe, prio, xcd_map, emaj, pace (see tvb_hpc.autotune); coupling 'gather' (CSR
+        pipe, prio, xcd_map, emaj, pace (see tvb_hpc.autotune); adj_unroll:
+        gathers in flight per group of the adjoint's transposed rows (0: the
+        forward's unroll); coupling 'gather' (CSR
         rows) or 'mfma' (delay-binned sums on the matrix cores; bfloat16 /
         float16 rings holding pre(y), blocked launches of at most 16 steps).
     """
@@ -442,7 +444,7 @@ class Simulation:
                  order='cuda', ipl=1, ring_dtype='f32', ring_scale=1.0,
                  adjoint=False, unroll=8, min_waves=1, pipe=False,
                  prio=True, xcd_map=False, coupling='gather', emaj=0,
-                 pace=False, context=None):
+                 pace=False, adj_unroll=0, context=None):
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.csr = csr
         N, P = csr.nnode, int(n_inst)
@@ -466,7 +468,8 @@ class Simulation:
                                     ring_pre=ring_pre, order=order, ipl=ipl,
                                     ring_dtype=ring_dtype, adjoint=adjoint,
                                     unroll=unroll, min_waves=min_waves,
-                                    pipe=pipe, prio=prio, emaj=emaj)
+                                    pipe=pipe, prio=prio, emaj=emaj,
+                                    adj_unroll=adj_unroll)
         if ring_scale != 1.0 and not self.spec.ring_pre:
             raise ValueError('ring_scale needs a ring that holds pre(y)')
         if ring_dtype == 'u8':
