@@ -431,7 +431,7 @@ class Simulation:
     launch: ipl, waves, node_block, steps_per_launch, unroll, min_waves,
         pipe, prio, xcd_map, emaj, pace (see tvb_hpc.autotune); adj_unroll:
         gathers in flight per group of the adjoint's transposed rows (0: the
-        forward's unroll); coupling 'gather' (CSR
+        forward's unroll; 32 allows at most 8 waves); coupling 'gather' (CSR
         rows) or 'mfma' (delay-binned sums on the matrix cores; bfloat16 /
         float16 rings holding pre(y), blocked launches of at most 16 steps).
     """
