@@ -13,6 +13,7 @@ import numpy as np
 
 from . import expr as _e
 
+
 def default_target():
     from .hip import HipTarget
     return HipTarget()
