@@ -209,6 +209,24 @@ def test_ring_pre_only_without_post_syn():
                            scheme.EulerStep(0.1)).ring_pre
 
 
+def test_adjoint_unroll_reaches_the_adjoint_source_only():
+    """adj_unroll sets the adjoint's gathers per group (0: the forward's
+    unroll); the forward-only source does not depend on it."""
+    m = model.MontbrioPazoRoxin()
+    c = coupling.Linear(m)
+    sc = scheme.EulerStep(0.1)
+
+    def spec(**kw):
+        return emit.KernelSpec(m, c, sc, unroll=16, **kw)
+    assert '#define TVBX_ADJ_UNROLL 16' in spec(adjoint=True).source()
+    assert '#define TVBX_ADJ_UNROLL 32' in spec(adjoint=True,
+                                                adj_unroll=32).source()
+    assert 'TVBX_ADJ_UNROLL' not in spec().source()
+    assert spec().key() == spec(adj_unroll=32).key()
+    with pytest.raises(ValueError, match='adj_unroll'):
+        spec(adjoint=True, adj_unroll=12)
+
+
 def test_scheme_dt_symbolic_needs_value():
     s = scheme.EulerStep(E.var('dt'))
     with pytest.raises(ValueError):
