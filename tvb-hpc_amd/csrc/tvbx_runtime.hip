@@ -589,7 +589,10 @@ k_fc_update(const float *__restrict__ samples, int slots, long long first,
   // FC_RG node rows of sample g / FC_GS
   const int sl = lane & 31, sh = lane >> 5;
   const int sg = 2 * wave + sh;
-  const int sk = sg / FC_GS, sr0 = (sg % FC_GS) * FC_RG;
+  // the sample a wave stages is the same for its two halves (FC_GS is
+  // even): sk = wave / 2
+  const int sk = __builtin_amdgcn_readfirstlane(sg / FC_GS);
+  const int sr0 = (sg % FC_GS) * FC_RG;
   const int snode0 = sr0 < FC_TILE ? I0 + sr0 : J0 + sr0 - FC_TILE;
   // the shifts of the tile's 32 nodes, staged once
 #pragma unroll
@@ -599,20 +602,49 @@ k_fc_update(const float *__restrict__ samples, int slots, long long first,
     xs[sl * FC_XS + FC_KC * 48 + r] =
         node < N ? shift[(size_t)node * pitch + p0 + sl] : 0.0f;
   }
+  __syncthreads();  // the staged shifts are read next
+  // the shifts of the operand nodes (row node on of the A operand, column
+  // node on of the B operand) of each of the wave's instances, in FP64
+  double sa[FC_IW], sb[FC_IW];
+#pragma unroll
+  for (int q = 0; q < FC_IW; ++q) {
+    const float *sq = xs + (FC_IW * wave + q) * FC_XS + FC_KC * 48 + on;
+    sa[q] = (double)sq[0];
+    sb[q] = (double)sq[16];
+  }
   fc_d4 acc[FC_IW];
 #pragma unroll
   for (int q = 0; q < FC_IW; ++q) acc[q] = fc_d4{0.0, 0.0, 0.0, 0.0};
   const int slot0 = (int)(first % slots);
+  // this lane's element of node row snode0 + t of a sample: 32-bit byte
+  // offsets from the sample's base (the host checks N pitch 4 < 4 GiB),
+  // through a buffer descriptor; rows past N read row N - 1 and stage 0
+  unsigned eo[FC_RG];
+#pragma unroll
+  for (int t = 0; t < FC_RG; ++t)
+    eo[t] = ((unsigned)min(snode0 + t, N - 1) * (unsigned)pitch +
+             (unsigned)(p0 + sl)) * 4u;
   float pre[FC_RG];
   auto fetch = [&](int k0) {
-    int slot = slot0 + k0 + sk;
-    slot -= slot >= slots ? slots : 0;
-    const bool live = k0 + sk < count;
-    const float *x = samples + (size_t)(live ? slot : 0) * NP + p0 + sl;
+    if (k0 + sk < count) {  // uniform: one sample per wave
+      int slot = slot0 + k0 + sk;
+      slot -= slot >= slots ? slots : 0;
+      const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+          const_cast<float *>(samples + (size_t)slot * NP), (short)0,
+          (int)0xffffffff, 0x00020000);
 #pragma unroll
-    for (int t = 0; t < FC_RG; ++t) {
-      const int node = snode0 + t;
-      pre[t] = (live && node < N) ? x[(size_t)node * pitch] : 0.0f;
+      for (int t = 0; t < FC_RG; ++t) {
+        const float v = __builtin_bit_cast(
+            float, __builtin_amdgcn_raw_buffer_load_b32(r, eo[t], 0, 0));
+        pre[t] = snode0 + t < N ? v : 0.0f;
+      }
+    } else {
+      // a sample past the block is staged as the rows' shifts, so that its
+      // d is exactly 0 (as for the padding nodes past N, whose values and
+      // shifts are both staged as 0)
+#pragma unroll
+      for (int t = 0; t < FC_RG; ++t)
+        pre[t] = xs[sl * FC_XS + FC_KC * 48 + sr0 + t];
     }
   };
   fetch(0);
@@ -623,14 +655,11 @@ k_fc_update(const float *__restrict__ samples, int slots, long long first,
       xs[sl * FC_XS + sk * 48 + sr0 + t] = pre[t];
     __syncthreads();
     if (k0 + FC_KC < count) fetch(k0 + FC_KC);  // in flight meanwhile
-    const bool kon = k0 + ok < count;
 #pragma unroll
     for (int q = 0; q < FC_IW; ++q) {
-      const float *xi = xs + (FC_IW * wave + q) * FC_XS;
-      const float *xq = xi + ok * 48 + on;
-      const float *sq = xi + FC_KC * 48 + on;
-      const double a = kon ? (double)xq[0] - (double)sq[0] : 0.0;
-      const double b = kon ? (double)xq[16] - (double)sq[16] : 0.0;
+      const float *xq = xs + (FC_IW * wave + q) * FC_XS + ok * 48 + on;
+      const double a = (double)xq[0] - sa[q];
+      const double b = (double)xq[16] - sb[q];
       acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
     }
   }
@@ -1513,6 +1542,8 @@ static int fc_flush(tvbx_sim *s) {
   const unsigned long long nwg =
       ((unsigned long long)n_tiles * n_fblk + 7) / 8 * 8;
   CHECK(nwg < (1ull << 31), "FC problem too large for one launch");
+  CHECK((unsigned long long)N * s->pitch * 4ull < (1ull << 32),
+        "one monitor sample must stay under 4 GiB for the FC merge");
   hipLaunchKernelGGL(k_fc_update, dim3((unsigned)nwg), dim3(64 * FC_NW), 0,
                      s->ctx->stream, (const float *)s->samples.p,
                      d.sample_slots, first, (int)pending, N, s->pitch, n_prev,
@@ -2184,6 +2215,8 @@ extern "C" int tvbx_corr_noalloc(tvbx_ctx *c, const float *samples,
   }
   hipLaunchKernelGGL(k_fc_prep, dim3(pitch / 64, N), dim3(64), 0, st, smp,
                      (int)n, 0ll, (int)n, N, pitch, 0.0, mean, shift, s1, u);
+  CHECK((unsigned long long)N * pitch * 4ull < (1ull << 32),
+        "one sample must stay under 4 GiB for the FC merge");
   hipLaunchKernelGGL(k_fc_update, dim3((unsigned)nwg), dim3(64 * FC_NW), 0,
                      st, smp, (int)n, 0ll, (int)n, N, pitch, 0.0, tiles,
                      n_tiles, n_fblk, shift, s1, u, m2);
@@ -2401,6 +2434,8 @@ extern "C" int tvbx_fc_loss(tvbx_ctx *c, const float *samples, long long n,
   }
   hipLaunchKernelGGL(k_fc_prep, dim3(pitch / 64, N), dim3(64), 0, st, smp,
                      (int)n, 0ll, (int)n, N, pitch, 0.0, dmean, shift, s1, u);
+  CHECK((unsigned long long)N * pitch * 4ull < (1ull << 32),
+        "one sample must stay under 4 GiB for the FC merge");
   hipLaunchKernelGGL(k_fc_update, dim3((unsigned)nwg), dim3(64 * FC_NW), 0,
                      st, smp, (int)n, 0ll, (int)n, N, pitch, 0.0, tiles,
                      n_tiles, n_fblk, shift, s1, u, m2);
