@@ -617,27 +617,26 @@ k_fc_update(const float *__restrict__ samples, int slots, long long first,
   for (int q = 0; q < FC_IW; ++q) acc[q] = fc_d4{0.0, 0.0, 0.0, 0.0};
   const int slot0 = (int)(first % slots);
   // this lane's element of node row snode0 + t of a sample: 32-bit byte
-  // offsets from the sample's base (the host checks N pitch 4 < 4 GiB),
-  // through a buffer descriptor; rows past N read row N - 1 and stage 0
-  unsigned eo[FC_RG];
-#pragma unroll
-  for (int t = 0; t < FC_RG; ++t)
-    eo[t] = ((unsigned)min(snode0 + t, N - 1) * (unsigned)pitch +
-             (unsigned)(p0 + sl)) * 4u;
+  // offsets from the sample's base (the host checks N pitch 4 < 4 GiB)
+  // through a buffer descriptor whose range is one sample, so that the
+  // padding rows past N read as 0 (a buffer load past the range returns 0)
+  const unsigned eo0 =
+      ((unsigned)snode0 * (unsigned)pitch + (unsigned)(p0 + sl)) * 4u;
+  const unsigned rowb = (unsigned)pitch * 4u;
+  const int nrec = (int)((unsigned)N * (unsigned)pitch * 4u);
   float pre[FC_RG];
   auto fetch = [&](int k0) {
     if (k0 + sk < count) {  // uniform: one sample per wave
       int slot = slot0 + k0 + sk;
       slot -= slot >= slots ? slots : 0;
       const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-          const_cast<float *>(samples + (size_t)slot * NP), (short)0,
-          (int)0xffffffff, 0x00020000);
+          const_cast<float *>(samples + (size_t)slot * NP), (short)0, nrec,
+          0x00020000);
 #pragma unroll
-      for (int t = 0; t < FC_RG; ++t) {
-        const float v = __builtin_bit_cast(
-            float, __builtin_amdgcn_raw_buffer_load_b32(r, eo[t], 0, 0));
-        pre[t] = snode0 + t < N ? v : 0.0f;
-      }
+      for (int t = 0; t < FC_RG; ++t)
+        pre[t] = __builtin_bit_cast(
+            float,
+            __builtin_amdgcn_raw_buffer_load_b32(r, eo0 + t * rowb, 0, 0));
     } else {
       // a sample past the block is staged as the rows' shifts, so that its
       // d is exactly 0 (as for the padding nodes past N, whose values and
@@ -655,11 +654,18 @@ k_fc_update(const float *__restrict__ samples, int slots, long long first,
       xs[sl * FC_XS + sk * 48 + sr0 + t] = pre[t];
     __syncthreads();
     if (k0 + FC_KC < count) fetch(k0 + FC_KC);  // in flight meanwhile
+    // all the wave's operand reads first, then the products
+    float xa[FC_IW], xb[FC_IW];
 #pragma unroll
     for (int q = 0; q < FC_IW; ++q) {
       const float *xq = xs + (FC_IW * wave + q) * FC_XS + ok * 48 + on;
-      const double a = (double)xq[0] - sa[q];
-      const double b = (double)xq[16] - sb[q];
+      xa[q] = xq[0];
+      xb[q] = xq[16];
+    }
+#pragma unroll
+    for (int q = 0; q < FC_IW; ++q) {
+      const double a = (double)xa[q] - sa[q];
+      const double b = (double)xb[q] - sb[q];
       acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
     }
   }
