@@ -191,6 +191,7 @@ def _emaj_allowed(cfg):
 def tune(cfg, steps=100, warmup=10, ctx=None, log=None,
          variants=(dict(unroll=16), dict(unroll=4), dict(min_waves=4),
                    dict(min_waves=4, unroll=4), dict(min_waves=4, unroll=16),
+                   dict(min_waves=2), dict(min_waves=2, unroll=16),
                    dict(xcd_map=True), dict(xcd_map=True, min_waves=4),
                    dict(pipe=True), dict(pipe=True, min_waves=4),
                    dict(pipe=True, min_waves=4, unroll=4),
